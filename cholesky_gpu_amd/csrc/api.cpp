@@ -1,0 +1,950 @@
+// api.cpp -- C ABI of libmpchol: handle, generators, factor and PCG drivers.
+//
+// The drivers are single-stream schedules of the kernels in k_*.hip:
+//   factor (switching_chol, switching_chol.cuh:76-224): per block column j
+//     POTRF(A_jj) -> TRSM(panel) + row stats -> decide + quantise -> SYRK(A22)
+//     with no host synchronisation inside the loop (the reference reads a
+//     device flag on the host every step, switching_chol.cuh:124).
+//   PCG (precond_CG, mixed_cholesky.cu:145-217): multi-RHS, FP64 vectors,
+//     device-resident alpha/beta, one small host read per iteration for the
+//     stopping test on the FP64 residual (confirmed on the true residual).
+#include "mpc_internal.hpp"
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+// ---------------------------------------------------------------- errors
+static thread_local char g_err[1024] = "";
+
+void mpc_set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char* mpc_last_error_string(void) { return g_err; }
+extern "C" const char* mpc_version_string(void) { return "mpchol 0.1 (gfx950)"; }
+
+int mpc_ws_reserve(mpc_workspace& w, size_t bytes) {
+    if (bytes <= w.bytes && w.ptr) return 0;
+    if (w.ptr) {
+        hipFree(w.ptr);
+        w.ptr = nullptr;
+        w.bytes = 0;
+    }
+    size_t b = bytes < 256 ? 256 : bytes;
+    hipError_t e = hipMalloc(&w.ptr, b);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        w.ptr = nullptr;
+        mpc_set_error("device allocation of %zu bytes failed: %s", b, hipGetErrorString(e));
+        return MPC_ERR_NOMEM;
+    }
+    w.bytes = b;
+    return 0;
+}
+
+
+// ---------------------------------------------------------------- small init kernels
+namespace {
+__global__ void init_small_kernel(mpc_small* sm) {
+    if (threadIdx.x == 0) {
+        sm->info = 0;
+        sm->nan_flag = 0;
+        sm->mu_bits = 0x7f800000u;
+    }
+    for (int t = threadIdx.x; t < 1024; t += blockDim.x) {
+        sm->tile_amax[t] = 0u;
+        sm->tile_maxrs[t] = 0u;
+        sm->tile_prec[t] = 0;
+    }
+}
+__global__ void init_factor_kernel(const float* A, int64_t lda, int64_t n, float* tracker,
+                                   uint8_t* map, int64_t nt) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) tracker[i] = A[i + i * lda];
+    if (map && i < nt * nt) {
+        int64_t r = i % nt, c = i / nt;
+        map[i] = r == c ? (uint8_t)MPC_PREC_FP32 : (uint8_t)255;
+    }
+}
+__global__ void fill_int_kernel(int* p, int n, int v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+}  // namespace
+
+// ---------------------------------------------------------------- handle
+extern "C" int mpc_handle_create(mpc_handle** out, int device) {
+    if (!out) { mpc_set_error("handle pointer is NULL"); return MPC_ERR_ARG; }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        (void)hipGetLastError();
+        mpc_set_error("no HIP device");
+        return MPC_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= ndev) { mpc_set_error("device %d out of range", device); return MPC_ERR_ARG; }
+    MPC_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    MPC_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        mpc_set_error("device %d is %s; libmpchol is built for gfx950 only", device, prop.gcnArchName);
+        return MPC_ERR_UNSUPPORTED;
+    }
+    mpc_handle* h = new mpc_handle();
+    h->device = device;
+    h->num_cu = prop.multiProcessorCount;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete h;
+        mpc_set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
+        return MPC_ERR_HIP;
+    }
+    h->own_stream = true;
+    int rc = mpc_ws_reserve(h->small, sizeof(mpc_small));
+    if (rc) { hipStreamDestroy(h->stream); delete h; return rc; }
+    *out = h;
+    return 0;
+}
+
+static void ws_free(mpc_workspace& w) {
+    if (w.ptr) hipFree(w.ptr);
+    w.ptr = nullptr;
+    w.bytes = 0;
+}
+
+extern "C" int mpc_handle_destroy(mpc_handle* h) {
+    if (!h) return 0;
+    hipSetDevice(h->device);
+    hipStreamSynchronize(h->stream);
+    for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
+                             &h->pcg, &h->scratch})
+        ws_free(*w);
+    if (h->own_stream) hipStreamDestroy(h->stream);
+    delete h;
+    return 0;
+}
+
+extern "C" int mpc_set_stream(mpc_handle* h, void* s) {
+    MPC_CHECK(h, "handle is NULL");
+    MPC_HIP(hipStreamSynchronize(h->stream));
+    if (h->own_stream) hipStreamDestroy(h->stream);
+    if (s) {
+        h->stream = (hipStream_t)s;
+        h->own_stream = false;
+    } else {
+        MPC_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->own_stream = true;
+    }
+    return 0;
+}
+
+extern "C" void* mpc_get_stream(mpc_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+extern "C" int mpc_synchronize(mpc_handle* h) {
+    MPC_CHECK(h, "handle is NULL");
+    MPC_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------- host RNG (matches device)
+void mpc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+static inline double host_u53(uint32_t a, uint32_t b) {
+    return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+double mpc_uniform_host(uint64_t seed, uint64_t stream, uint64_t idx) {
+    uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)stream, 0x554e4946u};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)}, o[4];
+    mpc_philox4x32(c, k, o);
+    return host_u53(o[0], o[1]);
+}
+double mpc_normal_host(uint64_t seed, uint64_t stream, uint64_t idx) {
+    uint64_t q = idx >> 1;
+    uint32_t c[4] = {(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)stream, 0x4e4f524du};
+    uint32_t k[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)}, o[4];
+    mpc_philox4x32(c, k, o);
+    double u1 = 1.0 - host_u53(o[0], o[1]), u2 = host_u53(o[2], o[3]);
+    double r = std::sqrt(-2.0 * std::log(u1)), th = 6.283185307179586 * u2;
+    return (idx & 1) ? r * std::sin(th) : r * std::cos(th);
+}
+
+// ---------------------------------------------------------------- matgen
+extern "C" void mpc_matgen_opts_default(mpc_matgen_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->kind = MPC_MAT_HOUSEHOLDER_SPD;
+    o->dist = MPC_DIST_LOGSPACED;
+    o->cond = 1e6;
+    o->lam_max = 1e6;
+    o->n_reflectors = 4;
+    o->seed = 20240531ull;
+}
+
+// spectrum: DistType semantics of construct_diag_* (kernels_macros.cuh:635-675)
+static void host_spectrum(const mpc_matgen_opts* o, int64_t n, std::vector<double>& lam) {
+    lam.resize(n);
+    int64_t nc = std::min<int64_t>(o->n_cluster, n);
+    for (int64_t i = 0; i < nc; ++i) {
+        double t = nc > 1 ? (double)i / (double)(nc - 1) : 0.0;
+        lam[i] = std::exp(std::log(o->cluster_lo) + t * (std::log(o->cluster_hi) - std::log(o->cluster_lo)));
+    }
+    int64_t m = n - nc;
+    for (int64_t i = 0; i < m; ++i) {
+        double t = m > 1 ? (double)i / (double)(m - 1) : 0.0, v;
+        switch (o->dist) {
+        case MPC_DIST_RANDOM_LOG_UNIFORM: {
+            double u = 1.0 - mpc_uniform_host(o->seed, 7, (uint64_t)i);
+            v = std::pow(10.0, std::log10(1.0 / o->cond) * (1.0 - u));
+            break; }
+        case MPC_DIST_CLUSTERED: v = i < m - 1 ? 1.0 : 1.0 / o->cond; break;
+        case MPC_DIST_ARITHMETIC: v = 1.0 - (1.0 - t) * (1.0 - 1.0 / o->cond); break;
+        case MPC_DIST_GEOMETRIC: v = std::pow(o->cond, -t); break;
+        default: v = std::exp(std::log(1.0 / o->cond) * (1.0 - t)); break;
+        }
+        lam[nc + i] = v * o->lam_max;
+    }
+}
+
+extern "C" int mpc_matgen(mpc_handle* h, float* dA, int64_t n, int64_t lda,
+                          const mpc_matgen_opts* o, double* d_lambda) {
+    MPC_CHECK(h && dA && o, "matgen: NULL argument");
+    MPC_CHECK(n >= 1 && lda >= n, "matgen: n=%lld lda=%lld", (long long)n, (long long)lda);
+    MPC_CHECK(o->kind == MPC_MAT_HOUSEHOLDER_SPD, "matgen: unknown kind %d", o->kind);
+    MPC_CHECK(o->n_reflectors >= 1 && o->n_reflectors <= 4, "matgen: n_reflectors must be 1..4");
+    MPC_CHECK(o->dist >= 0 && o->dist <= MPC_DIST_LOGSPACED, "matgen: unknown dist %d", o->dist);
+    MPC_CHECK(o->cond >= 1.0 && o->lam_max > 0.0, "matgen: cond >= 1 and lam_max > 0 required");
+    MPC_CHECK(o->n_cluster == 0 || (o->cluster_lo > 0 && o->cluster_hi >= o->cluster_lo),
+              "matgen: cluster bounds");
+    MPC_HIP(hipSetDevice(h->device));
+    std::vector<double> lam;
+    host_spectrum(o, n, lam);
+    const int r = o->n_reflectors;
+    std::vector<double> W(4 * n, 0.0), P(4 * n, 0.0), g(4 * n, 0.0);
+    for (int k = 0; k < r; ++k) {
+        double s = 0.0;
+        for (int64_t i = 0; i < n; ++i) {
+            double v = mpc_normal_host(o->seed, (uint64_t)k, (uint64_t)i);
+            W[4 * i + k] = v;
+            s += v * v;
+        }
+        double inv = 1.0 / std::sqrt(s);
+        for (int64_t i = 0; i < n; ++i) W[4 * i + k] *= inv;
+    }
+    double T[4][4] = {{0}};
+    for (int k = 0; k < r; ++k) {
+        double y[4] = {0};
+        for (int a = 0; a < k; ++a) {
+            double s = 0.0;
+            for (int64_t i = 0; i < n; ++i) s += W[4 * i + a] * W[4 * i + k];
+            y[a] = s;
+        }
+        for (int a = 0; a < k; ++a) {
+            double s = 0.0;
+            for (int b = a; b < k; ++b) s += T[a][b] * y[b];
+            T[a][k] = -2.0 * s;
+        }
+        T[k][k] = 2.0;
+    }
+    double G[4][4] = {{0}};
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+            double s = 0.0;
+            for (int64_t i = 0; i < n; ++i) s += W[4 * i + a] * lam[i] * W[4 * i + b];
+            G[a][b] = s;
+        }
+    for (int64_t i = 0; i < n; ++i)
+        for (int b = 0; b < 4; ++b) {
+            double s = 0.0;
+            for (int a = 0; a <= b; ++a) s += W[4 * i + a] * T[a][b];
+            P[4 * i + b] = s;
+        }
+    for (int64_t i = 0; i < n; ++i)
+        for (int a = 0; a < 4; ++a) {
+            double s = 0.0;
+            for (int b = 0; b < 4; ++b) s += G[a][b] * P[4 * i + b];
+            g[4 * i + a] = s;
+        }
+    // device copies: lam | W | P | g
+    size_t bytes = sizeof(double) * (size_t)n * 13;
+    int rc = mpc_ws_reserve(h->scratch, bytes);
+    if (rc) return rc;
+    double* d = (double*)h->scratch.ptr;
+    double *dl = d, *dW = d + n, *dP = d + 5 * n, *dg = d + 9 * n;
+    MPC_HIP(hipMemcpyAsync(dl, lam.data(), sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    MPC_HIP(hipMemcpyAsync(dW, W.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, h->stream));
+    MPC_HIP(hipMemcpyAsync(dP, P.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, h->stream));
+    MPC_HIP(hipMemcpyAsync(dg, g.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, h->stream));
+    rc = mpc_launch_hh_matrix(h->stream, n, dl, dW, dP, dg, dA, lda);
+    if (rc) return rc;
+    if (d_lambda)
+        MPC_HIP(hipMemcpyAsync(d_lambda, dl, sizeof(double) * n, hipMemcpyDeviceToDevice, h->stream));
+    MPC_HIP(hipStreamSynchronize(h->stream));   // host vectors go out of scope
+    return 0;
+}
+
+// ---------------------------------------------------------------- RBF
+extern "C" int mpc_rbf_points(mpc_handle* h, float* dX, int64_t n, int d, int64_t ldx, int kind,
+                              double p_zero, uint64_t seed) {
+    MPC_CHECK(h && dX, "rbf_points: NULL argument");
+    MPC_CHECK(n >= 1 && d >= 1 && ldx >= d, "rbf_points: bad shape");
+    MPC_CHECK(kind == 0 || kind == 1, "rbf_points: kind must be 0 or 1");
+    MPC_HIP(hipSetDevice(h->device));
+    return mpc_launch_points(h->stream, dX, n, d, ldx, kind, p_zero, seed);
+}
+
+extern "C" int mpc_rbf_assemble(mpc_handle* h, const float* dX, int64_t n, int d, int64_t ldx,
+                                const double* ls, int n_ls, double variance, double nugget,
+                                float* dK, int64_t ldk) {
+    MPC_CHECK(h && dX && ls && dK, "rbf_assemble: NULL argument");
+    MPC_CHECK(n >= 1 && d >= 1 && ldx >= d && ldk >= n, "rbf_assemble: bad shape");
+    MPC_CHECK(n_ls == 1 || n_ls == d, "rbf_assemble: n_ls must be 1 or d");
+    for (int i = 0; i < n_ls; ++i) MPC_CHECK(ls[i] > 0, "rbf_assemble: lengthscales must be > 0");
+    MPC_HIP(hipSetDevice(h->device));
+    std::vector<float> inv(n_ls);
+    for (int i = 0; i < n_ls; ++i) inv[i] = (float)(1.0 / ls[i]);
+    int rc = mpc_ws_reserve(h->scratch, sizeof(float) * n_ls + 256);
+    if (rc) return rc;
+    MPC_HIP(hipMemcpyAsync(h->scratch.ptr, inv.data(), sizeof(float) * n_ls, hipMemcpyHostToDevice,
+                           h->stream));
+    rc = mpc_launch_rbf(h->stream, dX, n, d, ldx, (const double*)h->scratch.ptr, n_ls, variance,
+                        nugget, dK, ldk);
+    if (rc) return rc;
+    MPC_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int mpc_median_pairwise_distance(mpc_handle* h, const float* dX, int64_t n, int d,
+                                            int64_t ldx, int64_t nsub, uint64_t seed, double* out) {
+    MPC_CHECK(h && dX && out, "median_pairwise_distance: NULL argument");
+    MPC_CHECK(nsub >= 2 && nsub <= n, "median_pairwise_distance: nsub must be in [2, n]");
+    MPC_HIP(hipSetDevice(h->device));
+    // seeded partial Fisher-Yates on the host picks the subsample
+    std::vector<int64_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    for (int64_t i = 0; i < nsub; ++i) {
+        int64_t j = i + (int64_t)(mpc_uniform_host(seed, 11, (uint64_t)i) * (double)(n - i));
+        if (j >= n) j = n - 1;
+        std::swap(idx[i], idx[j]);
+    }
+    std::vector<float> Xs((size_t)nsub * d), row(d);
+    MPC_HIP(hipStreamSynchronize(h->stream));
+    for (int64_t i = 0; i < nsub; ++i)
+        MPC_HIP(hipMemcpy(Xs.data() + i * d, dX + idx[i] * ldx, sizeof(float) * d,
+                          hipMemcpyDeviceToHost));
+    std::vector<double> dist;
+    dist.reserve((size_t)nsub * (nsub - 1) / 2);
+    for (int64_t i = 0; i < nsub; ++i)
+        for (int64_t j = i + 1; j < nsub; ++j) {
+            double s = 0.0;
+            for (int c = 0; c < d; ++c) {
+                double t = (double)Xs[i * d + c] - (double)Xs[j * d + c];
+                s += t * t;
+            }
+            dist.push_back(std::sqrt(s));
+        }
+    size_t mid = dist.size() / 2;
+    std::nth_element(dist.begin(), dist.begin() + mid, dist.end());
+    double med = dist[mid];
+    if (dist.size() % 2 == 0) {
+        double lo = *std::max_element(dist.begin(), dist.begin() + mid);
+        med = 0.5 * (med + lo);
+    }
+    *out = med;
+    return 0;
+}
+
+// ---------------------------------------------------------------- factor
+extern "C" void mpc_factor_opts_default(mpc_factor_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->nb = 256;
+    o->update_prec = MPC_PREC_SWITCH;
+    o->ladder[0] = MPC_PREC_E4M3;
+    o->ladder[1] = MPC_PREC_FP32;
+    o->n_ladder = 2;
+    o->eps_prime = 0.25;
+    o->fallback = 1;
+    o->profile = 0;
+}
+
+static int validate_factor_opts(const mpc_factor_opts* o) {
+    MPC_CHECK(o->nb == 64 || o->nb == 128 || o->nb == 256, "factor: nb must be 64, 128 or 256");
+    if (o->update_prec == MPC_PREC_SWITCH) {
+        MPC_CHECK(o->n_ladder >= 1 && o->n_ladder <= 4, "factor: n_ladder must be 1..4");
+        bool f16 = false, other16 = false;
+        for (int i = 0; i < o->n_ladder; ++i) {
+            MPC_CHECK(mpc_valid_prec(o->ladder[i]), "factor: ladder[%d] = %d invalid", i, o->ladder[i]);
+            if (o->ladder[i] == MPC_PREC_FP16) f16 = true;
+            if (o->ladder[i] == MPC_PREC_BF16 || mpc_is_fp8(o->ladder[i])) other16 = true;
+        }
+        MPC_CHECK(!(f16 && other16), "factor: FP16 can only be combined with FP32 in a ladder");
+        MPC_CHECK(o->eps_prime > 0, "factor: eps_prime must be > 0");
+    } else {
+        MPC_CHECK(mpc_valid_prec(o->update_prec), "factor: update_prec %d invalid", o->update_prec);
+    }
+    return 0;
+}
+
+static inline int tile_rows_for(int nb) { return nb < 128 ? 128 : nb; }
+
+void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* class_mask,
+                    unsigned* fp8_mask) {
+    int precs[4];
+    int np = 0;
+    if (update_prec >= 0) precs[np++] = update_prec;
+    else
+        for (int i = 0; i < n_ladder && i < 4; ++i) precs[np++] = ladder[i];
+    auto cls = [](int p, int q) {
+        if (p == MPC_PREC_FP32 || q == MPC_PREC_FP32) return MPC_PREC_FP32;
+        if (p == MPC_PREC_BF16 || q == MPC_PREC_BF16) return MPC_PREC_BF16;
+        if (p == MPC_PREC_FP16 || q == MPC_PREC_FP16) return MPC_PREC_FP16;
+        return MPC_PREC_E4M3;
+    };
+    unsigned cm = 0, fm = 0;
+    for (int a = 0; a < np; ++a) {
+        for (int b = 0; b < np; ++b) cm |= 1u << cls(precs[a], precs[b]);
+        if (precs[a] == MPC_PREC_E4M3 || precs[a] == MPC_PREC_MXE4M3) fm |= 1u;
+        if (precs[a] == MPC_PREC_E5M2 || precs[a] == MPC_PREC_MXE5M2) fm |= 2u;
+    }
+    *class_mask = cm;
+    *fp8_mask = fm;
+}
+
+// one factorisation attempt; A in place
+static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const mpc_factor_opts* o,
+                       double eps, uint8_t* map, float* tracker, mpc_factor_info* fi) {
+    const int nb = o->nb;
+    const int64_t nt = (n + nb - 1) / nb;
+    const int tr = tile_rows_for(nb);
+    hipStream_t st = h->stream;
+    int rc;
+    if ((rc = mpc_ws_reserve(h->panel, (size_t)(n + 128) * nb * 4))) return rc;
+    if ((rc = mpc_ws_reserve(h->scales, (size_t)(n + 128) * (nb / 32)))) return rc;
+    if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nt * nb / 32 + 8) * 1024 * sizeof(float)))) return rc;
+    if ((rc = mpc_ws_reserve(h->rowstat, (size_t)(n + 128) * sizeof(float)))) return rc;
+    mpc_small* sm = (mpc_small*)h->small.ptr;
+    uint8_t* payload = (uint8_t*)h->panel.ptr;
+    uint8_t* scales = (uint8_t*)h->scales.ptr;
+    float* dinv = (float*)h->dinv.ptr;
+    float* rowss = (float*)h->rowstat.ptr;
+    const int ladder[4] = {o->ladder[0], o->ladder[1], o->ladder[2], o->ladder[3]};
+    unsigned cmask = 0, fmask = 0;
+    mpc_syrk_masks(o->update_prec, ladder, o->n_ladder, &cmask, &fmask);
+
+    hipLaunchKernelGGL(init_small_kernel, dim3(1), dim3(256), 0, st, sm);
+    int64_t ninit = std::max<int64_t>(n, nt * nt);
+    hipLaunchKernelGGL(init_factor_kernel, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st,
+                       dA, lda, n, tracker, map, nt);
+    MPC_LAUNCH_CHECK();
+
+    hipEvent_t ev[6] = {};
+    const bool prof = o->profile != 0;
+    double tp = 0, tt = 0, tq = 0, ts = 0;
+    if (prof)
+        for (auto& e : ev) MPC_HIP(hipEventCreate(&e));
+    hipEvent_t e0, e1;
+    MPC_HIP(hipEventCreate(&e0));
+    MPC_HIP(hipEventCreate(&e1));
+    MPC_HIP(hipEventRecord(e0, st));
+    for (int64_t j = 0; j < nt; ++j) {
+        const int64_t c0 = j * nb, bs = std::min<int64_t>(nb, n - c0), e = c0 + bs, s = n - e;
+        const int ntl = (int)((s + tr - 1) / tr);
+        float* Ajj = dA + c0 + c0 * lda;
+        if (prof) MPC_HIP(hipEventRecord(ev[0], st));
+        if ((rc = mpc_launch_potrf(st, Ajj, lda, (int)bs, (int)c0, dinv, sm, ntl))) return rc;
+        if (prof) MPC_HIP(hipEventRecord(ev[1], st));
+        if (s == 0) {
+            if (prof) {
+                MPC_HIP(hipEventSynchronize(ev[1]));
+                float ms;
+                hipEventElapsedTime(&ms, ev[0], ev[1]);
+                tp += ms;
+            }
+            break;
+        }
+        float* A21 = dA + e + c0 * lda;
+        if ((rc = mpc_launch_trsm_stats(st, A21, lda, s, nb, dinv + (c0 / 32) * 1024, Ajj,
+                                        rowss + e, tracker + e, tr, sm)))
+            return rc;
+        if (prof) MPC_HIP(hipEventRecord(ev[2], st));
+        if ((rc = mpc_launch_quantize_ex(st, A21, s, nb, lda, tr, nullptr, sm->tile_amax, sm,
+                                         o->update_prec, ladder, o->n_ladder, eps, sm->tile_prec,
+                                         map ? map + (j + 1) + j * nt : nullptr, payload,
+                                         (int64_t)nb * 4, scales, &sm->nan_flag)))
+            return rc;
+        if (prof) MPC_HIP(hipEventRecord(ev[3], st));
+        if ((rc = mpc_launch_syrk_classes(st, dA + e + e * lda, s, lda, nb, payload,
+                                          (int64_t)nb * 4, scales, tr, sm->tile_prec, sm,
+                                          cmask, fmask)))
+            return rc;
+        if (prof) {
+            MPC_HIP(hipEventRecord(ev[4], st));
+            MPC_HIP(hipEventSynchronize(ev[4]));
+            float ms;
+            hipEventElapsedTime(&ms, ev[0], ev[1]); tp += ms;
+            hipEventElapsedTime(&ms, ev[1], ev[2]); tt += ms;
+            hipEventElapsedTime(&ms, ev[2], ev[3]); tq += ms;
+            hipEventElapsedTime(&ms, ev[3], ev[4]); ts += ms;
+        }
+    }
+    MPC_HIP(hipEventRecord(e1, st));
+    mpc_small hs;
+    MPC_HIP(hipMemcpyAsync(&hs, sm, offsetof(mpc_small, tile_amax), hipMemcpyDeviceToHost, st));
+    MPC_HIP(hipStreamSynchronize(st));
+    float ms_total = 0;
+    hipEventElapsedTime(&ms_total, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (prof)
+        for (auto& e : ev) hipEventDestroy(e);
+    fi->ms_total += ms_total;
+    fi->ms_potrf += tp;
+    fi->ms_trsm += tt;
+    fi->ms_quant += tq;
+    fi->ms_syrk += ts;
+    fi->min_pivot = (double)__builtin_bit_cast(float, hs.mu_bits);
+    fi->info = hs.info;
+    if (hs.nan_flag && !hs.info) {
+        mpc_set_error("factor: NaN/Inf (or FP16 overflow) in a panel");
+        return MPC_ERR_NAN;
+    }
+    return hs.info;
+}
+
+static void flops_from_map(const uint8_t* map, int64_t n, int nb, mpc_factor_info* fi) {
+    const int64_t nt = (n + nb - 1) / nb;
+    const int tr = tile_rows_for(nb);
+    for (int k = 0; k < 8; ++k) { fi->syrk_flops[k] = 0; fi->tiles[k] = 0; }
+    auto cls = [](int p, int q) {
+        if (p == MPC_PREC_FP32 || q == MPC_PREC_FP32) return MPC_PREC_FP32;
+        if (p == MPC_PREC_BF16 || q == MPC_PREC_BF16) return MPC_PREC_BF16;
+        if (p == MPC_PREC_FP16 || q == MPC_PREC_FP16) return MPC_PREC_FP16;
+        return MPC_PREC_E4M3;
+    };
+    for (int64_t j = 0; j < nt; ++j) {
+        const int64_t e = std::min<int64_t>((j + 1) * nb, n), s = n - e;
+        if (s <= 0) break;
+        const int64_t ntl = (s + tr - 1) / tr;
+        std::vector<int> p(ntl);
+        for (int64_t t = 0; t < ntl; ++t) {
+            p[t] = map[(j + 1 + t) + j * nt];
+            if (p[t] < 8) fi->tiles[p[t]]++;
+        }
+        const double k = (double)nb;
+        for (int64_t a = 0; a < ntl; ++a) {
+            double ra = (double)std::min<int64_t>(tr, s - a * tr);
+            for (int64_t b = 0; b <= a; ++b) {
+                double rb = (double)std::min<int64_t>(tr, s - b * tr);
+                double f = a == b ? ra * (ra + 1) * k : 2.0 * ra * rb * k;
+                fi->syrk_flops[cls(p[a], p[b])] += f;
+            }
+        }
+    }
+}
+
+static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const mpc_factor_opts* o,
+                       uint8_t* d_map, float* d_tracker, mpc_factor_info* info,
+                       const float* restore_from, int64_t restore_ld) {
+    MPC_CHECK(h && dA, "factor: NULL argument");
+    MPC_CHECK(n >= 1 && lda >= n, "factor: n=%lld lda=%lld", (long long)n, (long long)lda);
+    mpc_factor_opts def;
+    if (!o) { mpc_factor_opts_default(&def); o = &def; }
+    int rc = validate_factor_opts(o);
+    if (rc) return rc;
+    MPC_HIP(hipSetDevice(h->device));
+    const int nb = o->nb;
+    const int64_t nt = (n + nb - 1) / nb;
+    MPC_CHECK(((n + tile_rows_for(nb) - 1) / tile_rows_for(nb)) <= 1024,
+              "factor: n too large for the tile statistics (max 1024 tiles)");
+    mpc_factor_info fi;
+    std::memset(&fi, 0, sizeof fi);
+    // internal map / tracker when the caller does not want them
+    size_t need = (size_t)nt * nt + 256 + (d_tracker ? 0 : (size_t)(n + 64) * sizeof(float));
+    uint8_t* map = d_map;
+    float* tracker = d_tracker;
+    if (!map || !tracker) {
+        if ((rc = mpc_ws_reserve(h->scratch, need))) return rc;
+        uint8_t* base = (uint8_t*)h->scratch.ptr;
+        if (!tracker) tracker = (float*)base;
+        if (!map) map = base + (d_tracker ? 0 : (size_t)(n + 64) * sizeof(float));
+    }
+    // backup for the fallback path when the caller gave no source to restore from
+    const float* src = restore_from;
+    int64_t src_ld = restore_ld;
+    if (o->fallback && !src) {
+        if ((rc = mpc_ws_reserve(h->amat, sizeof(float) * (size_t)n * n))) return rc;
+        if ((rc = mpc_launch_copy_f32(h->stream, n, n, dA, lda, (float*)h->amat.ptr, n))) return rc;
+        src = (const float*)h->amat.ptr;
+        src_ld = n;
+    }
+    double eps = o->eps_prime;
+    mpc_factor_opts cur = *o;
+    int attempt = 0;
+    for (;;) {
+        rc = factor_once(h, dA, n, lda, &cur, eps, map, tracker, &fi);
+        if (rc <= 0 || !o->fallback) break;
+        // non-positive pivot: restore A, then retry with a smaller eps_prime
+        // (twice), then with every update in FP32; an FP32 failure is final
+        const bool all_fp32 = cur.update_prec == MPC_PREC_FP32;
+        if (all_fp32) break;
+        int rc2 = mpc_launch_copy_f32(h->stream, n, n, src, src_ld, dA, lda);
+        if (rc2) return rc2;
+        ++attempt;
+        if (cur.update_prec == MPC_PREC_SWITCH && attempt <= 2) eps /= 16.0;
+        else cur.update_prec = MPC_PREC_FP32;
+    }
+    fi.retries = attempt;
+    fi.eps_prime_used = eps;
+    const double nd = (double)n;
+    fi.flops_total = nd * nd * nd / 3.0 + nd * nd / 2.0 + nd / 6.0;
+    if (rc >= 0) {
+        std::vector<uint8_t> hm((size_t)nt * nt);
+        hipError_t e = hipMemcpy(hm.data(), map, hm.size(), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) { mpc_set_error("map copy: %s", hipGetErrorString(e)); return MPC_ERR_HIP; }
+        flops_from_map(hm.data(), n, nb, &fi);
+    }
+    if (info) *info = fi;
+    // the dinv blocks now describe this factor
+    h->dinv_owner = dA;
+    h->dinv_n = n;
+    h->dinv_ld = lda;
+    return rc;
+}
+
+extern "C" int mpc_factor(mpc_handle* h, float* dA, int64_t n, int64_t lda,
+                          const mpc_factor_opts* o, uint8_t* d_prec_map, float* d_updated_diag,
+                          mpc_factor_info* info) {
+    return factor_impl(h, dA, n, lda, o, d_prec_map, d_updated_diag, info, nullptr, 0);
+}
+
+// ---------------------------------------------------------------- PCG
+extern "C" void mpc_pcg_opts_default(mpc_pcg_opts* o) {
+    if (!o) return;
+    o->tol = 1e-10;
+    o->maxit = 500;
+    o->stop_rule = 0;
+    o->x0_mode = 0;
+}
+
+namespace {
+struct pcg_bufs {
+    double *R, *Z, *P, *Q, *W, *Y, *part;
+    double *rz, *rz_new, *pq, *rr, *bb;
+    int* active;
+};
+
+// z = L^-T L^-1 r  (all columns)
+int apply_prec(mpc_handle* h, const float* L, int64_t ldl, int64_t n, int nb, int nrhs,
+               const double* R, double* Z, pcg_bufs& b) {
+    hipStream_t st = h->stream;
+    const float* dinv = (const float*)h->dinv.ptr;
+    int rc = mpc_launch_copy2d(st, n, nrhs, R, n, b.W, n);
+    if (rc) return rc;
+    const int64_t nt = (n + nb - 1) / nb;
+    for (int64_t j = 0; j < nt; ++j)
+        if ((rc = mpc_launch_trsv_step_fwd(st, L, ldl, n, nb, (int)j, dinv, b.W, n, nrhs, b.Y, n)))
+            return rc;
+    for (int64_t j = nt - 1; j >= 0; --j)
+        if ((rc = mpc_launch_trsv_step_bwd(st, L, ldl, n, nb, (int)j, dinv, b.Y, n, nrhs, Z, n)))
+            return rc;
+    return 0;
+}
+}  // namespace
+
+static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_t ldl,
+                    int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
+                    int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info) {
+    MPC_CHECK(h && dA && dL && dB && dX, "pcg: NULL argument");
+    MPC_CHECK(n >= 1 && lda >= n && ldl >= n && ldb >= n && ldx >= n, "pcg: bad leading dimension");
+    MPC_CHECK(nrhs >= 1 && nrhs <= 256, "pcg: nrhs must be 1..256");
+    MPC_CHECK(nb == 64 || nb == 128 || nb == 256, "pcg: nb must be 64, 128 or 256");
+    mpc_pcg_opts def;
+    if (!o) { mpc_pcg_opts_default(&def); o = &def; }
+    MPC_CHECK(o->tol > 0 && o->maxit >= 0, "pcg: tol > 0 and maxit >= 0 required");
+    MPC_HIP(hipSetDevice(h->device));
+    hipStream_t st = h->stream;
+    int rc;
+    // diagonal-block inverses of L
+    const int64_t nsb = (n + 31) / 32;
+    if (h->dinv_owner != dL || h->dinv_n != n || h->dinv_ld != ldl) {
+        if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nsb + 8) * 1024 * sizeof(float)))) return rc;
+        if ((rc = mpc_launch_dinv_from_L(st, dL, ldl, n, (float*)h->dinv.ptr))) return rc;
+        h->dinv_owner = dL;
+        h->dinv_n = n;
+        h->dinv_ld = ldl;
+    }
+    const int ns = mpc_gemv_nsplit(n);
+    const size_t vec = (size_t)n * nrhs;
+    const size_t part = std::max<size_t>((size_t)ns * vec, (size_t)nrhs * ((n + 2047) / 2048) + 64);
+    const size_t nbytes = sizeof(double) * (6 * vec + part + 5 * (size_t)nrhs + 64) + sizeof(int) * nrhs + 1024;
+    if ((rc = mpc_ws_reserve(h->pcg, nbytes))) return rc;
+    pcg_bufs b;
+    double* base = (double*)h->pcg.ptr;
+    b.R = base; b.Z = b.R + vec; b.P = b.Z + vec; b.Q = b.P + vec; b.W = b.Q + vec; b.Y = b.W + vec;
+    b.part = b.Y + vec;
+    b.rz = b.part + part; b.rz_new = b.rz + nrhs; b.pq = b.rz_new + nrhs; b.rr = b.pq + nrhs;
+    b.bb = b.rr + nrhs;
+    b.active = (int*)(b.bb + nrhs + 8);
+
+    hipEvent_t e0, e1;
+    MPC_HIP(hipEventCreate(&e0));
+    MPC_HIP(hipEventCreate(&e1));
+    MPC_HIP(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(fill_int_kernel, dim3((nrhs + 255) / 256), dim3(256), 0, st, b.active, nrhs, 1);
+    // B into a packed copy for the norms
+    if ((rc = mpc_launch_dot(st, n, nrhs, dB, ldb, dB, ldb, b.bb, b.part))) return rc;
+    // x0
+    if (o->x0_mode == 0) {
+        double* tmpX = b.Z;    // M^-1 B, then copy to X
+        if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.R, n))) return rc;
+        if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, tmpX, b))) return rc;
+        if ((rc = mpc_launch_copy2d(st, n, nrhs, tmpX, n, dX, ldx))) return rc;
+    } else if (o->x0_mode == 1) {
+        if ((rc = mpc_launch_axpby(st, n, nrhs, 0.0, dX, ldx, 0.0, dX, ldx))) return rc;
+    }
+    // R = B - A X
+    if ((rc = mpc_launch_gemv(st, dA, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
+    if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.R, n))) return rc;
+    if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.R, n))) return rc;
+    if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return rc;
+    if ((rc = mpc_launch_copy2d(st, n, nrhs, b.Z, n, b.P, n))) return rc;
+    if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.Z, n, b.rz, b.part))) return rc;
+    if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.R, n, b.rr, b.part))) return rc;
+
+    std::vector<double> hb(nrhs), hrr(nrhs), hrz(nrhs), hpq(nrhs), hrzn(nrhs);
+    std::vector<int> act(nrhs, 1), iters(nrhs, 0);
+    std::vector<double> relres(nrhs, 0.0), alphas, betas;
+    MPC_HIP(hipMemcpyAsync(hb.data(), b.bb, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    MPC_HIP(hipMemcpyAsync(hrr.data(), b.rr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    MPC_HIP(hipMemcpyAsync(hrz.data(), b.rz, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    MPC_HIP(hipStreamSynchronize(st));
+    std::vector<double> nb2(nrhs);
+    int nactive = 0;
+    for (int j = 0; j < nrhs; ++j) {
+        nb2[j] = std::sqrt(hb[j]);
+        if (nb2[j] == 0.0) nb2[j] = 1.0;
+        double res = o->stop_rule == 1 ? std::sqrt(std::fabs(hrz[j])) / nb2[j] : std::sqrt(hrr[j]) / nb2[j];
+        relres[j] = res;
+        if (res <= o->tol) act[j] = 0; else ++nactive;
+    }
+    bool act_dirty = nactive != nrhs;
+    int k = 0;
+    while (nactive > 0 && k < o->maxit) {
+        ++k;
+        if (act_dirty) {
+            MPC_HIP(hipMemcpyAsync(b.active, act.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+            act_dirty = false;
+        }
+        if ((rc = mpc_launch_gemv(st, dA, lda, n, nrhs, b.P, n, b.Q, n, b.part))) return rc;
+        if ((rc = mpc_launch_dot(st, n, nrhs, b.P, n, b.Q, n, b.pq, b.part))) return rc;
+        if ((rc = mpc_launch_pcg_update_xr(st, n, nrhs, dX, ldx, b.R, n, b.P, n, b.Q, n, b.rz, b.pq,
+                                           b.active)))
+            return rc;
+        if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return rc;
+        if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.Z, n, b.rz_new, b.part))) return rc;
+        if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.R, n, b.rr, b.part))) return rc;
+        if ((rc = mpc_launch_pcg_update_p(st, n, nrhs, b.P, n, b.Z, n, b.rz_new, b.rz, b.active))) return rc;
+        MPC_HIP(hipMemcpyAsync(hrr.data(), b.rr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+        MPC_HIP(hipMemcpyAsync(hrzn.data(), b.rz_new, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+        MPC_HIP(hipMemcpyAsync(hpq.data(), b.pq, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+        // rz <- rz_new for active columns (device copy; inactive columns keep theirs)
+        if ((rc = mpc_launch_copy2d(st, nrhs, 1, b.rz_new, nrhs, b.rz, nrhs))) return rc;
+        MPC_HIP(hipStreamSynchronize(st));
+        if (act[0]) {
+            alphas.push_back(hrz[0] / hpq[0]);
+            betas.push_back(hrzn[0] / hrz[0]);
+        }
+        bool need_true = false;
+        std::vector<int> cand(nrhs, 0);
+        for (int j = 0; j < nrhs; ++j) {
+            if (!act[j]) continue;
+            iters[j] = k;
+            double res = o->stop_rule == 1 ? std::sqrt(std::fabs(hrzn[j])) / nb2[j] : std::sqrt(hrr[j]) / nb2[j];
+            relres[j] = res;
+            if (res <= o->tol) { cand[j] = 1; need_true = true; }
+        }
+        for (int j = 0; j < nrhs; ++j) hrz[j] = act[j] ? hrzn[j] : hrz[j];
+        if (need_true && o->stop_rule == 0) {
+            // confirm on the true residual B - A X; replace R where it lags
+            if ((rc = mpc_launch_gemv(st, dA, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
+            if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.W, n))) return rc;
+            if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.W, n))) return rc;
+            if ((rc = mpc_launch_dot(st, n, nrhs, b.W, n, b.W, n, b.rr, b.part))) return rc;
+            MPC_HIP(hipMemcpyAsync(hrr.data(), b.rr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+            MPC_HIP(hipStreamSynchronize(st));
+            for (int j = 0; j < nrhs; ++j) {
+                if (!cand[j]) continue;
+                double tr = std::sqrt(hrr[j]) / nb2[j];
+                relres[j] = tr;
+                if (tr <= o->tol) {
+                    act[j] = 0;
+                    --nactive;
+                    act_dirty = true;
+                } else {
+                    // residual replacement for column j
+                    MPC_HIP(hipMemcpyAsync(b.R + (size_t)j * n, b.W + (size_t)j * n, sizeof(double) * n,
+                                           hipMemcpyDeviceToDevice, st));
+                }
+            }
+        } else if (need_true) {
+            for (int j = 0; j < nrhs; ++j)
+                if (cand[j]) { act[j] = 0; --nactive; act_dirty = true; }
+        }
+    }
+    MPC_HIP(hipEventRecord(e1, st));
+    MPC_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    mpc_pcg_info pi;
+    std::memset(&pi, 0, sizeof pi);
+    pi.iters = *std::max_element(iters.begin(), iters.end());
+    pi.converged = 0;
+    for (int j = 0; j < nrhs; ++j) pi.converged += act[j] ? 0 : 1;
+    pi.relres = *std::max_element(relres.begin(), relres.end());
+    pi.ms_total = ms;
+    pi.kappa_est = 1.0;
+    if (!alphas.empty()) {
+        // Lanczos tridiagonal from CG coefficients; extreme eigenvalues by bisection
+        int m = (int)alphas.size();
+        std::vector<double> dg(m), off2(m > 1 ? m - 1 : 1);
+        for (int i = 0; i < m; ++i) {
+            dg[i] = 1.0 / alphas[i] + (i > 0 ? betas[i - 1] / alphas[i - 1] : 0.0);
+            if (i > 0) off2[i - 1] = betas[i - 1] / (alphas[i - 1] * alphas[i - 1]);
+        }
+        auto count = [&](double x) {
+            int c = 0;
+            double q = dg[0] - x;
+            if (q < 0) ++c;
+            for (int i = 1; i < m; ++i) {
+                if (q == 0.0) q = 1e-300;
+                q = dg[i] - x - off2[i - 1] / q;
+                if (q < 0) ++c;
+            }
+            return c;
+        };
+        double lo = 1e300, hi = -1e300;
+        for (int i = 0; i < m; ++i) {
+            double rad = (i > 0 ? std::sqrt(off2[i - 1]) : 0) + (i < m - 1 ? std::sqrt(off2[i]) : 0);
+            lo = std::min(lo, dg[i] - rad);
+            hi = std::max(hi, dg[i] + rad);
+        }
+        double a = lo, bq = hi;
+        for (int it = 0; it < 200; ++it) { double mm = 0.5 * (a + bq); (count(mm) >= 1 ? bq : a) = mm; }
+        double emin = 0.5 * (a + bq);
+        a = lo; bq = hi;
+        for (int it = 0; it < 200; ++it) { double mm = 0.5 * (a + bq); (count(mm) >= m ? bq : a) = mm; }
+        double emax = 0.5 * (a + bq);
+        pi.kappa_est = emin > 0 ? emax / emin : INFINITY;
+    }
+    if (info) *info = pi;
+    return pi.converged == nrhs ? 0 : MPC_ERR_NOT_CONVERGED;
+}
+
+extern "C" int mpc_pcg(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_t ldl,
+                       int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
+                       int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info) {
+    return pcg_impl(h, dA, lda, dL, ldl, n, nb, nrhs, dB, ldb, dX, ldx, o, info);
+}
+
+extern "C" int mpc_precond_cg(mpc_handle* h, const float* dA, int64_t lda, int64_t n, int nrhs,
+                              const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                              const mpc_factor_opts* fo, const mpc_pcg_opts* po,
+                              mpc_factor_info* finfo, mpc_pcg_info* pinfo) {
+    MPC_CHECK(h && dA && dB && dX, "precond_cg: NULL argument");
+    MPC_CHECK(n >= 1 && lda >= n, "precond_cg: bad shape");
+    MPC_HIP(hipSetDevice(h->device));
+    mpc_factor_opts def;
+    if (!fo) { mpc_factor_opts_default(&def); fo = &def; }
+    int rc = mpc_ws_reserve(h->amat, sizeof(float) * (size_t)n * n);
+    if (rc) return rc;
+    float* L = (float*)h->amat.ptr;
+    if ((rc = mpc_launch_copy_f32(h->stream, n, n, dA, lda, L, n))) return rc;
+    // the caller's A stays intact, so the fallback restores from it
+    rc = factor_impl(h, L, n, n, fo, nullptr, nullptr, finfo, dA, lda);
+    if (rc != 0) return rc;
+    return pcg_impl(h, dA, lda, L, n, n, fo->nb, nrhs, dB, ldb, dX, ldx, po, pinfo);
+}
+
+// ---------------------------------------------------------------- building blocks
+extern "C" int mpc_quantize_panel(mpc_handle* h, const float* dP, int64_t m, int64_t k, int64_t ld,
+                                  int tile_rows, const uint8_t* d_prec_of_tile, uint8_t* d_payload,
+                                  int64_t row_stride, uint8_t* d_scales) {
+    MPC_CHECK(h && dP && d_prec_of_tile && d_payload && d_scales, "quantize_panel: NULL argument");
+    MPC_CHECK(m >= 1 && k >= 32 && k % 32 == 0 && ld >= m, "quantize_panel: bad shape");
+    MPC_CHECK(tile_rows >= 32 && tile_rows % 32 == 0, "quantize_panel: tile_rows multiple of 32");
+    MPC_CHECK(row_stride >= 4 * k, "quantize_panel: row_stride must be >= 4*k");
+    MPC_HIP(hipSetDevice(h->device));
+    const int64_t nt = (m + tile_rows - 1) / tile_rows;
+    int rc = mpc_ws_reserve(h->scratch, sizeof(uint32_t) * nt + sizeof(int) * 4 + 256);
+    if (rc) return rc;
+    uint32_t* amax = (uint32_t*)h->scratch.ptr;
+    int* nanf = (int*)(amax + nt);
+    MPC_HIP(hipMemsetAsync(h->scratch.ptr, 0, sizeof(uint32_t) * nt + sizeof(int) * 4, h->stream));
+    if ((rc = mpc_launch_tile_amax(h->stream, dP, m, (int)k, ld, tile_rows, amax))) return rc;
+    if ((rc = mpc_launch_quantize_ex(h->stream, dP, m, (int)k, ld, tile_rows, d_prec_of_tile, amax,
+                                     nullptr, 0, nullptr, 0, 0.0, nullptr, nullptr, d_payload,
+                                     row_stride, d_scales, nanf)))
+        return rc;
+    int hn = 0;
+    MPC_HIP(hipMemcpyAsync(&hn, nanf, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    MPC_HIP(hipStreamSynchronize(h->stream));
+    if (hn) { mpc_set_error("quantize_panel: NaN/Inf or FP16 overflow in the panel"); return MPC_ERR_NAN; }
+    return 0;
+}
+
+extern "C" int mpc_syrk(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_t k,
+                        const uint8_t* d_payload, int64_t row_stride, const uint8_t* d_scales,
+                        int tile_rows, const uint8_t* d_prec_of_tile) {
+    MPC_CHECK(h && dC && d_payload && d_scales && d_prec_of_tile, "syrk: NULL argument");
+    MPC_CHECK(m >= 1 && ldc >= m && k >= 64 && k % 64 == 0, "syrk: bad shape (k multiple of 64)");
+    MPC_CHECK(tile_rows >= 128 && tile_rows % 128 == 0, "syrk: tile_rows multiple of 128");
+    MPC_CHECK(row_stride >= 4 * k, "syrk: row_stride must be >= 4*k");
+    MPC_HIP(hipSetDevice(h->device));
+    return mpc_launch_syrk(h->stream, dC, m, ldc, (int)k, d_payload, row_stride, d_scales, tile_rows,
+                           d_prec_of_tile, nullptr);
+}
+
+extern "C" int mpc_potrf_block(mpc_handle* h, float* dA, int n, int64_t lda, int* info) {
+    MPC_CHECK(h && dA, "potrf_block: NULL argument");
+    MPC_CHECK(n >= 1 && n <= 256 && lda >= n, "potrf_block: n must be 1..256");
+    MPC_HIP(hipSetDevice(h->device));
+    int rc = mpc_ws_reserve(h->dinv, (size_t)16 * 1024 * sizeof(float));
+    if (rc) return rc;
+    mpc_small* sm = (mpc_small*)h->small.ptr;
+    hipLaunchKernelGGL(init_small_kernel, dim3(1), dim3(256), 0, h->stream, sm);
+    if ((rc = mpc_launch_potrf(h->stream, dA, lda, n, 0, (float*)h->dinv.ptr, sm, 0))) return rc;
+    int hi = 0;
+    MPC_HIP(hipMemcpyAsync(&hi, &sm->info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    MPC_HIP(hipStreamSynchronize(h->stream));
+    h->dinv_owner = nullptr;
+    if (info) *info = hi;
+    return hi;
+}
+
+extern "C" int mpc_convert_f8(mpc_handle* h, const float* dx, uint8_t* dy, int64_t count, int prec) {
+    MPC_CHECK(h && dx && dy, "convert_f8: NULL argument");
+    MPC_CHECK(prec == MPC_PREC_E4M3 || prec == MPC_PREC_E5M2, "convert_f8: prec must be E4M3 or E5M2");
+    MPC_CHECK(count >= 0, "convert_f8: count");
+    MPC_HIP(hipSetDevice(h->device));
+    if (count == 0) return 0;
+    return mpc_launch_convert_f8(h->stream, dx, dy, count, prec);
+}

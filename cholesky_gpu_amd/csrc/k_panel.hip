@@ -1,0 +1,273 @@
+// k_panel.hip -- panel TRSM with fused row statistics, per-tile precision
+// decision, and the fused tile quantiser.
+//
+//  trsm_stats: L21 = A21 * L_jj^{-T} (the RIGHT/LOWER/TRANS form of
+//    fp8_chol.cuh:73-85; switching_chol.cuh:98-110 calls a LEFT form that is
+//    out of bounds).  One wave owns a 32-row slab and substitutes over the
+//    32-column blocks of the panel with the inverses of L_jj's 32x32 diagonal
+//    sub-blocks (MFMA 32x32x2 f32).  The epilogue computes each row's sum of
+//    squares, subtracts it from the Schur-diagonal tracker (update_diag,
+//    kernels_macros.cuh:177-190) and folds the tile amax / max row sum of
+//    squares into per-tile slots with order-independent atomics.
+//  quantize: the panel becomes the SYRK operand: row-major k-contiguous
+//    payload plus one E8M0 byte per row per 32 columns (replaces
+//    convert_fp32_to_mxe4m3 / convert_transpose_fp32_to_mxe4m3,
+//    fp8_utils.cuh:27-130, and convertFloatToHalf, kernels_macros.cuh:115-124).
+//    Each workgroup decides its tile's precision from the tile statistics;
+//    all workgroups of a tile evaluate the same expression on the same inputs.
+#include "mpc_internal.hpp"
+#include "mpc_formats.hpp"
+
+namespace {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ int sigma(int s, int h) { return 8 * (s >> 2) + 4 * h + (s & 3); }
+
+template <int NBK>
+__global__ __launch_bounds__(256) void trsm_stats_kernel(float* __restrict__ A21, int64_t lda,
+                                                         int64_t s, const float* __restrict__ dinv_j,
+                                                         const float* __restrict__ Ljj,
+                                                         float* __restrict__ rowss,
+                                                         float* __restrict__ tracker, int tile_rows,
+                                                         mpc_small* __restrict__ sm) {
+    if (sm->info != 0) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * 32;
+    if (r0 >= s) return;
+    const int n = lane & 31, h = lane >> 5;
+    const int64_t row = r0 + n;
+    const bool valid = row < s;
+    v16f X[NBK];
+#pragma unroll
+    for (int b = 0; b < NBK; ++b) {
+        v16f acc;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            int q = sigma(i, h);
+            acc[i] = valid ? A21[row + (int64_t)(32 * b + q) * lda] : 0.f;
+        }
+#pragma unroll
+        for (int p = 0; p < b; ++p) {
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                float a = -Ljj[(32 * b + n) + (int64_t)(32 * p + sigma(t, h)) * lda];
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, X[p][t], acc, 0, 0, 0);
+            }
+        }
+        v16f xb = {};
+        const float* Db = dinv_j + b * 1024;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            float a = Db[n + sigma(t, h) * 32];
+            xb = __builtin_amdgcn_mfma_f32_32x32x2f32(a, acc[t], xb, 0, 0, 0);
+        }
+        X[b] = xb;
+        if (valid) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) A21[row + (int64_t)(32 * b + sigma(i, h)) * lda] = xb[i];
+        }
+    }
+    float ss = 0.f, am = 0.f;
+#pragma unroll
+    for (int b = 0; b < NBK; ++b)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            ss = fmaf(X[b][i], X[b][i], ss);
+            am = fmaxf(am, fabsf(X[b][i]));
+        }
+    ss += __shfl_xor(ss, 32);
+    am = fmaxf(am, __shfl_xor(am, 32));
+    if (!valid) { ss = 0.f; am = 0.f; }
+    if (valid && h == 0) {
+        rowss[row] = ss;
+        tracker[row] -= ss;
+    }
+    if (!(ss == ss) || !(am <= 3.4e38f)) atomicOr(&sm->nan_flag, 1);
+    // wave max (all 32 rows of the slab are in one tile: tile_rows % 32 == 0)
+    float wss = ss, wam = am;
+#pragma unroll
+    for (int o = 16; o >= 1; o >>= 1) {
+        wss = fmaxf(wss, __shfl_xor(wss, o));
+        wam = fmaxf(wam, __shfl_xor(wam, o));
+    }
+    if (lane == 0) {
+        int t = (int)(r0 / tile_rows);
+        atomicMax(&sm->tile_maxrs[t], __float_as_uint(wss));
+        atomicMax(&sm->tile_amax[t], __float_as_uint(wam));
+    }
+}
+
+// tile amax for the exported quantiser (no TRSM in front of it)
+__global__ __launch_bounds__(256) void tile_amax_kernel(const float* __restrict__ P, int64_t m,
+                                                        int k, int64_t ld, int tile_rows,
+                                                        uint32_t* __restrict__ tile_amax) {
+    const int64_t r0 = (int64_t)blockIdx.x * 32;
+    const int r = threadIdx.x & 31, qq = threadIdx.x >> 5;
+    const int64_t row = r0 + r;
+    float am = 0.f;
+    if (row < m)
+        for (int c = qq; c < k; c += 8) am = fmaxf(am, fabsf(P[row + (int64_t)c * ld]));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) am = fmaxf(am, __shfl_xor(am, o));
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = am;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float v = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        atomicMax(&tile_amax[r0 / tile_rows], __float_as_uint(v));
+    }
+}
+
+struct rule_args {
+    int update_prec;
+    int n_ladder;
+    int ladder[4];
+    double eps_prime;
+};
+
+__device__ __forceinline__ int decide(const rule_args& ra, float maxrs, float mu) {
+    if (ra.update_prec >= 0) return ra.update_prec;
+    for (int q = 0; q < ra.n_ladder; ++q)
+        if (mpc_dev_unit_roundoff(ra.ladder[q]) * (double)maxrs <= ra.eps_prime * (double)mu)
+            return ra.ladder[q];
+    return ra.ladder[ra.n_ladder - 1];
+}
+
+// One workgroup = 32 rows x k columns; thread (r, qq) owns row r, MX block qq.
+__global__ __launch_bounds__(256) void quantize_kernel(
+    const float* __restrict__ P, int64_t m, int k, int64_t ld, int tile_rows,
+    const uint8_t* __restrict__ prec_in, const uint32_t* __restrict__ tile_amax,
+    const uint32_t* __restrict__ tile_maxrs, const mpc_small* __restrict__ smc, rule_args ra,
+    uint8_t* __restrict__ prec_out, uint8_t* __restrict__ map_col, uint8_t* __restrict__ payload,
+    int64_t row_stride, uint8_t* __restrict__ scales, int* __restrict__ nan_flag) {
+    if (smc && smc->info != 0) return;
+    const int64_t r0 = (int64_t)blockIdx.x * 32;
+    const int r = threadIdx.x & 31, qq0 = threadIdx.x >> 5;
+    const int64_t row = r0 + r;
+    const int tile = (int)(r0 / tile_rows);
+    int prec;
+    if (prec_in) {
+        prec = prec_in[tile];
+    } else {
+        float mu = __uint_as_float(smc->mu_bits);
+        prec = decide(ra, __uint_as_float(tile_maxrs[tile]), mu);
+        if (r0 % tile_rows == 0 && threadIdx.x == 0) {
+            if (prec_out) prec_out[tile] = (uint8_t)prec;
+            if (map_col) map_col[tile] = (uint8_t)prec;
+        }
+    }
+    if (row >= m) return;
+    const int kb = k >> 5;
+    const float tam = __uint_as_float(tile_amax[tile]);
+    const int esz = mpc_dev_elem_bytes(prec);
+    uint8_t* prow = payload + row * row_stride;
+    bool bad = false;
+    for (int qq = qq0; qq < kb; qq += 8) {
+        float v[32];
+        float am = 0.f;
+#pragma unroll
+        for (int c = 0; c < 32; ++c) {
+            v[c] = P[row + (int64_t)(32 * qq + c) * ld];
+            am = fmaxf(am, fabsf(v[c]));
+        }
+        if (!(am <= 3.4e38f)) bad = true;
+        int e = 0;
+        if (prec == MPC_PREC_MXE4M3 || prec == MPC_PREC_MXE5M2) e = mpc_dev_scale_exp(am, prec);
+        else if (prec == MPC_PREC_E4M3 || prec == MPC_PREC_E5M2) e = mpc_dev_scale_exp(tam, prec);
+        scales[row * kb + qq] = (uint8_t)(e + 127);
+        uint8_t* dst = prow + (int64_t)32 * qq * esz;
+        if (prec == MPC_PREC_FP32) {
+#pragma unroll
+            for (int c = 0; c < 32; c += 4)
+                *reinterpret_cast<float4*>(dst + 4 * c) = make_float4(v[c], v[c + 1], v[c + 2], v[c + 3]);
+        } else if (prec == MPC_PREC_BF16 || prec == MPC_PREC_FP16) {
+            uint32_t w[16];
+#pragma unroll
+            for (int c = 0; c < 32; c += 2) {
+                uint32_t lo = prec == MPC_PREC_BF16 ? mpc_dev_f32_to_bf16(v[c]) : mpc_dev_f32_to_f16(v[c]);
+                uint32_t hi = prec == MPC_PREC_BF16 ? mpc_dev_f32_to_bf16(v[c + 1]) : mpc_dev_f32_to_f16(v[c + 1]);
+                w[c >> 1] = lo | (hi << 16);
+                if (prec == MPC_PREC_FP16 && ((lo & 0x7c00u) == 0x7c00u || (hi & 0x7c00u) == 0x7c00u)) bad = true;
+            }
+#pragma unroll
+            for (int c = 0; c < 16; c += 4)
+                *reinterpret_cast<uint4*>(dst + 4 * c) = make_uint4(w[c], w[c + 1], w[c + 2], w[c + 3]);
+        } else {
+            uint32_t w[8];
+#pragma unroll
+            for (int c = 0; c < 32; c += 4) {
+                float a0 = ldexpf(v[c], -e), a1 = ldexpf(v[c + 1], -e);
+                float a2 = ldexpf(v[c + 2], -e), a3 = ldexpf(v[c + 3], -e);
+                w[c >> 2] = mpc_dev_pack4_f8(a0, a1, a2, a3, prec);
+            }
+            *reinterpret_cast<uint4*>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
+            *reinterpret_cast<uint4*>(dst + 16) = make_uint4(w[4], w[5], w[6], w[7]);
+        }
+    }
+    if (bad) atomicOr(nan_flag, 1);
+}
+
+}  // namespace
+
+int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, int nb,
+                          const float* dinv_j, const float* Ljj, float* rowss, float* tracker,
+                          int tile_rows, mpc_small* sm) {
+    if (s <= 0) return 0;
+    if (tile_rows % 32) {
+        mpc_set_error("trsm: tile_rows %d not a multiple of 32", tile_rows);
+        return MPC_ERR_ARG;
+    }
+    dim3 grid((unsigned)((s + 127) / 128));
+    switch (nb) {
+    case 64:
+        hipLaunchKernelGGL(trsm_stats_kernel<2>, grid, dim3(256), 0, st, A21, lda, s, dinv_j, Ljj,
+                           rowss, tracker, tile_rows, sm);
+        break;
+    case 128:
+        hipLaunchKernelGGL(trsm_stats_kernel<4>, grid, dim3(256), 0, st, A21, lda, s, dinv_j, Ljj,
+                           rowss, tracker, tile_rows, sm);
+        break;
+    case 256:
+        hipLaunchKernelGGL(trsm_stats_kernel<8>, grid, dim3(256), 0, st, A21, lda, s, dinv_j, Ljj,
+                           rowss, tracker, tile_rows, sm);
+        break;
+    default:
+        mpc_set_error("trsm: unsupported block size %d", nb);
+        return MPC_ERR_ARG;
+    }
+    MPC_LAUNCH_CHECK();
+    return 0;
+}
+
+int mpc_launch_tile_amax(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
+                         int tile_rows, uint32_t* tile_amax) {
+    if (m <= 0) return 0;
+    hipLaunchKernelGGL(tile_amax_kernel, dim3((unsigned)((m + 31) / 32)), dim3(256), 0, st, P, m,
+                       k, ld, tile_rows, tile_amax);
+    MPC_LAUNCH_CHECK();
+    return 0;
+}
+
+// factor path: decision from the tile statistics in sm; exported path: prec_in.
+int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
+                           int tile_rows, const uint8_t* prec_in, const uint32_t* tile_amax,
+                           mpc_small* sm, int update_prec, const int* ladder, int n_ladder,
+                           double eps_prime, uint8_t* prec_out, uint8_t* map_col,
+                           uint8_t* payload, int64_t row_stride, uint8_t* scales, int* nan_flag) {
+    if (m <= 0) return 0;
+    if (k % 32 || tile_rows % 32) {
+        mpc_set_error("quantize: k (%d) and tile_rows (%d) must be multiples of 32", k, tile_rows);
+        return MPC_ERR_ARG;
+    }
+    rule_args ra;
+    ra.update_prec = update_prec;
+    ra.n_ladder = n_ladder;
+    for (int i = 0; i < 4; ++i) ra.ladder[i] = i < n_ladder ? ladder[i] : 0;
+    ra.eps_prime = eps_prime;
+    hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)((m + 31) / 32)), dim3(256), 0, st, P, m, k,
+                       ld, tile_rows, prec_in, tile_amax, sm ? sm->tile_maxrs : nullptr, sm, ra,
+                       prec_out, map_col, payload, row_stride, scales, nan_flag);
+    MPC_LAUNCH_CHECK();
+    return 0;
+}

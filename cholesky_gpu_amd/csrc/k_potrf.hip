@@ -1,0 +1,228 @@
+// k_potrf.hip -- FP32 Cholesky of one diagonal block (bs <= 256) in a single
+// workgroup, plus the inverses of its 32x32 diagonal sub-blocks.
+//
+// Replaces the cusolverDnSpotrf call of switching_chol.cuh:86-95 (CPU form:
+// micro_cholesky, micro_chol.hpp:14-44).  The block lives in LDS as 32x32
+// sub-tiles (lower triangle only, 36 sub-tiles x 32x33 floats = 152 KB for
+// bs = 256); the algorithm is right-looking over 32-column sub-panels:
+//   1. wave 0 factors the diagonal sub-tile in registers (row per lane) and
+//      inverts it (column per lane);
+//   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32);
+//   3. the trailing sub-tiles receive the rank-32 update (MFMA 32x32x2 f32).
+// Outputs: L in place (lower; the strict upper triangle of A is untouched),
+// dinv (column-major 32x32 inverse per sub-block), the running minimum pivot
+// squared and, on a non-positive pivot, the 1-based failing column.
+#include "mpc_internal.hpp"
+
+namespace {
+
+constexpr int SUBLD = 33;                 // padded sub-tile column stride
+constexpr int SUBSZ = 32 * SUBLD;         // floats per sub-tile
+constexpr int MAXSB = 8;                  // 256 / 32
+constexpr int NSUB = MAXSB * (MAXSB + 1) / 2;
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ int sidx(int bi, int bj) { return bi * (bi + 1) / 2 + bj; }
+
+__device__ __forceinline__ float rdlane(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// D[m][n] (32x32) += A[m][k] * B[k][n] over k = 0..31 with both operands in LDS:
+// A(m,k) = a[m + k*lda_], B(k,n) = b[n + k*ldb_] (i.e. B stored as rows n).
+__device__ __forceinline__ v16f mfma_32x32x32_lds(v16f acc, const float* a, int lda_,
+                                                 const float* b, int ldb_, float asign) {
+    const int lane = threadIdx.x & 63;
+    const int m = lane & 31, h = lane >> 5;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        const int k = 2 * s + h;
+        float av = asign * a[m + k * lda_];
+        float bv = b[m + k * ldb_];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64_t lda, int bs,
+                                                    int col0, float* __restrict__ dinv,
+                                                    mpc_small* __restrict__ sm, int ntiles_next) {
+    __shared__ float T[NSUB * SUBSZ];
+    __shared__ float Dinv[SUBSZ];
+    __shared__ int fail_col;
+    __shared__ float red[256];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (sm->info != 0) return;                       // an earlier step failed
+    for (int t = tid; t < ntiles_next; t += 256) {
+        sm->tile_amax[t] = 0u;
+        sm->tile_maxrs[t] = 0u;
+    }
+    if (tid == 0) fail_col = -1;
+    const int nsb = (bs + 31) >> 5;
+
+    // ---- load lower sub-tiles (padding: identity beyond bs)
+    for (int bi = 0; bi < nsb; ++bi)
+        for (int bj = 0; bj <= bi; ++bj) {
+            float* S = T + sidx(bi, bj) * SUBSZ;
+            for (int idx = tid; idx < 1024; idx += 256) {
+                int r = idx & 31, c = idx >> 5;
+                int gr = bi * 32 + r, gc = bj * 32 + c;
+                float v;
+                if (gr < bs && gc < bs)
+                    v = (gr >= gc) ? A[gr + (int64_t)gc * lda] : 0.f;
+                else
+                    v = (gr == gc) ? 1.f : 0.f;
+                S[r + c * SUBLD] = v;
+            }
+        }
+    __syncthreads();
+
+    for (int kb = 0; kb < nsb; ++kb) {
+        float* D = T + sidx(kb, kb) * SUBSZ;
+        // ---- 1. factor + invert the diagonal sub-tile (wave 0)
+        if (wave == 0) {
+            const int r = lane & 31;
+            float d[32];
+#pragma unroll
+            for (int c = 0; c < 32; ++c) d[c] = (c <= r) ? D[r + c * SUBLD] : 0.f;
+            int bad = -1;
+#pragma unroll
+            for (int k = 0; k < 32; ++k) {
+                float piv = rdlane(d[k], k);
+                if (!(piv > 0.f)) {                  // uniform across the wave
+                    if (bad < 0) bad = k;
+                    piv = 1.f;
+                }
+                float l = sqrtf(piv);
+                if (r == k) d[k] = l;
+                else if (r > k) d[k] = d[k] / l;
+#pragma unroll
+                for (int c = k + 1; c < 32; ++c) {
+                    float lck = rdlane(d[k], c);
+                    if (r >= c) d[c] = fmaf(-d[k], lck, d[c]);
+                }
+            }
+            if (lane < 32) {
+#pragma unroll
+                for (int c = 0; c < 32; ++c)
+                    if (c <= r) D[r + c * SUBLD] = d[c];
+            }
+            if (bad >= 0 && lane == 0 && kb * 32 + bad < bs && fail_col < 0)
+                fail_col = kb * 32 + bad;
+            // inverse, column per lane: X = L^{-1}, x[p] = (e_c[p] - sum L[p][q] x[q]) / L[p][p]
+            const int c = lane & 31;
+            float x[32];
+#pragma unroll
+            for (int p = 0; p < 32; ++p) {
+                float acc = (p == c) ? 1.f : 0.f;
+#pragma unroll
+                for (int q = 0; q < p; ++q) acc = fmaf(-rdlane(d[q], p), x[q], acc);
+                x[p] = acc / rdlane(d[p], p);
+            }
+            if (lane < 32) {
+                float* gd = dinv + (int64_t)((col0 >> 5) + kb) * 1024;
+#pragma unroll
+                for (int p = 0; p < 32; ++p) {
+                    Dinv[p + c * SUBLD] = x[p];
+                    if (kb * 32 < bs) gd[p + c * 32] = x[p];
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- 2. sub-tiles below: S <- S * Dinv^T   (out(r,c) = sum_q S(r,q) Dinv(c,q))
+        for (int ib = kb + 1 + wave; ib < nsb; ib += 4) {
+            float* S = T + sidx(ib, kb) * SUBSZ;
+            v16f acc = {};
+            // A(m=r,k=q) = S[r + q*ld]; B(k=q,n=c) = Dinv(c,q) = Dinv[c + q*ld]
+            acc = mfma_32x32x32_lds(acc, S, SUBLD, Dinv, SUBLD, 1.f);
+            const int n = lane & 31, h = lane >> 5;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                int m = 8 * (i >> 2) + 4 * h + (i & 3);
+                S[m + n * SUBLD] = acc[i];
+            }
+        }
+        __syncthreads();
+
+        // ---- 3. trailing update: S(ib,jb) -= S(ib,kb) * S(jb,kb)^T, kb < jb <= ib
+        {
+            const int nrem = nsb - 1 - kb;
+            const int npair = nrem * (nrem + 1) / 2;
+            for (int pi = wave; pi < npair; pi += 4) {
+                // decode pair index -> (ii, jj), jj <= ii, in units above kb
+                int ii = 0;
+                while ((ii + 1) * (ii + 2) / 2 <= pi) ++ii;
+                int jj = pi - ii * (ii + 1) / 2;
+                int ib = kb + 1 + ii, jb = kb + 1 + jj;
+                float* Cs = T + sidx(ib, jb) * SUBSZ;
+                const float* Ai = T + sidx(ib, kb) * SUBSZ;
+                const float* Bj = T + sidx(jb, kb) * SUBSZ;
+                const int n = lane & 31, h = lane >> 5;
+                v16f acc;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    int m = 8 * (i >> 2) + 4 * h + (i & 3);
+                    acc[i] = Cs[m + n * SUBLD];
+                }
+                // A(m=r,k=q) = -Ai(r,q); B(k=q,n=c) = Bj(c,q)
+                acc = mfma_32x32x32_lds(acc, Ai, SUBLD, Bj, SUBLD, -1.f);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    int m = 8 * (i >> 2) + 4 * h + (i & 3);
+                    Cs[m + n * SUBLD] = acc[i];
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- write back the lower triangle, min pivot^2, failure
+    float pmin = INFINITY;
+    for (int bi = 0; bi < nsb; ++bi)
+        for (int bj = 0; bj <= bi; ++bj) {
+            const float* S = T + sidx(bi, bj) * SUBSZ;
+            for (int idx = tid; idx < 1024; idx += 256) {
+                int r = idx & 31, c = idx >> 5;
+                int gr = bi * 32 + r, gc = bj * 32 + c;
+                if (gr < bs && gc < bs && gr >= gc) {
+                    float v = S[r + c * SUBLD];
+                    A[gr + (int64_t)gc * lda] = v;
+                    if (gr == gc) pmin = fminf(pmin, v * v);
+                }
+            }
+        }
+    red[tid] = pmin;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) red[tid] = fminf(red[tid], red[tid + w]);
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (fail_col >= 0) {
+            sm->info = col0 + fail_col + 1;
+        } else {
+            atomicMin(&sm->mu_bits, __float_as_uint(red[0]));
+        }
+    }
+}
+
+}  // namespace
+
+int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0, float* dinv,
+                     mpc_small* sm, int ntiles_next) {
+    if (bs <= 0 || bs > 256) {
+        mpc_set_error("potrf: block size %d outside 1..256", bs);
+        return MPC_ERR_ARG;
+    }
+    if (ntiles_next > 1024) {
+        mpc_set_error("potrf: %d panel tiles exceed 1024", ntiles_next);
+        return MPC_ERR_ARG;
+    }
+    hipLaunchKernelGGL(potrf_kernel, dim3(1), dim3(256), 0, st, A, lda, bs, col0, dinv, sm,
+                       ntiles_next);
+    MPC_LAUNCH_CHECK();
+    return 0;
+}

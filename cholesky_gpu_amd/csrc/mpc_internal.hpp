@@ -1,0 +1,162 @@
+// mpc_internal.hpp -- shared declarations of libmpchol (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <vector>
+#include "../../include/mpchol.h"
+
+// ---------------------------------------------------------------- errors
+void mpc_set_error(const char* fmt, ...);
+
+#define MPC_HIP(call)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (call);                                                \
+        if (e_ != hipSuccess) {                                                \
+            mpc_set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #call,   \
+                          hipGetErrorString(e_));                              \
+            return MPC_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+#define MPC_CHECK(cond, ...)                                                   \
+    do {                                                                       \
+        if (!(cond)) {                                                         \
+            mpc_set_error(__VA_ARGS__);                                        \
+            return MPC_ERR_ARG;                                                \
+        }                                                                      \
+    } while (0)
+
+#define MPC_LAUNCH_CHECK()                                                     \
+    do {                                                                       \
+        hipError_t e_ = hipGetLastError();                                     \
+        if (e_ != hipSuccess) {                                                \
+            mpc_set_error("%s:%d: kernel launch failed: %s", __FILE__, __LINE__,\
+                          hipGetErrorString(e_));                              \
+            return MPC_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
+// ---------------------------------------------------------------- precision
+// unit roundoff u_p of each storage/compute format
+inline double mpc_unit_roundoff(int p) {
+    switch (p) {
+    case MPC_PREC_FP32: return 5.9604644775390625e-08;   // 2^-24
+    case MPC_PREC_BF16: return 3.90625e-03;              // 2^-8
+    case MPC_PREC_FP16: return 4.8828125e-04;            // 2^-11
+    case MPC_PREC_E5M2: case MPC_PREC_MXE5M2: return 0.125;
+    default: return 0.0625;                              // E4M3: 2^-4
+    }
+}
+inline bool mpc_is_fp8(int p) { return p >= MPC_PREC_E5M2 && p <= MPC_PREC_MXE5M2; }
+inline bool mpc_valid_prec(int p) { return p >= 0 && p <= MPC_PREC_MXE5M2; }
+
+// ---------------------------------------------------------------- handle
+struct mpc_workspace {
+    void*  ptr = nullptr;
+    size_t bytes = 0;
+};
+
+struct mpc_handle {
+    int          device = 0;
+    hipStream_t  stream = nullptr;
+    bool         own_stream = false;
+    // factor workspace
+    mpc_workspace panel;      // quantised panel payload, n * nb * 4 bytes
+    mpc_workspace scales;     // n * nb/32 bytes
+    mpc_workspace dinv;       // inverses of 32x32 diagonal sub-blocks, n * 32 floats
+    mpc_workspace small;      // per-step scalars and tile stats
+    mpc_workspace rowstat;    // row sum of squares (n floats) + tracker (n floats)
+    mpc_workspace amat;       // copy of A for precond_cg / fallback
+    mpc_workspace pcg;        // PCG vectors
+    mpc_workspace scratch;    // generic
+    // provenance of the dinv blocks: the factor they belong to
+    const float* dinv_owner = nullptr;
+    int64_t      dinv_n = 0;
+    int64_t      dinv_ld = 0;
+    int          num_cu = 0;
+};
+
+// grow-only device allocation in a handle workspace
+int mpc_ws_reserve(mpc_workspace& w, size_t bytes);
+
+// ---------------------------------------------------------------- small device state
+// layout of handle->small (all device):
+struct mpc_small {
+    int      info;            // 1-based failing column, 0 = ok
+    int      nan_flag;        // NaN/Inf seen in a panel
+    uint32_t mu_bits;         // min pivot^2 so far (bits of a positive float; atomicMin)
+    uint32_t pad;
+    // per-step tile stats (index by tile within the panel), max 1024 tiles
+    uint32_t tile_amax[1024];   // bits of positive floats (atomicMax)
+    uint32_t tile_maxrs[1024];  // max row sum of squares
+    uint8_t  tile_prec[1024];   // decided precision per panel tile
+};
+
+// ---------------------------------------------------------------- kernels (launchers)
+// k_potrf.hip
+int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0,
+                     float* dinv, mpc_small* sm, int ntiles_next);
+// k_panel.hip
+int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, int nb,
+                          const float* dinv_j, const float* Ljj, float* rowss, float* tracker,
+                          int tile_rows, mpc_small* sm);
+int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
+                           int tile_rows, const uint8_t* prec_in, const uint32_t* tile_amax,
+                           mpc_small* sm, int update_prec, const int* ladder, int n_ladder,
+                           double eps_prime, uint8_t* prec_out, uint8_t* map_col,
+                           uint8_t* payload, int64_t row_stride, uint8_t* scales, int* nan_flag);
+int mpc_launch_tile_amax(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
+                         int tile_rows, uint32_t* tile_amax);
+// k_syrk.hip
+int mpc_launch_syrk(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
+                    const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
+                    int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm);
+int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
+                            const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
+                            int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
+                            unsigned class_mask, unsigned fp8_mask);
+// compute classes / FP8 element formats that a factor configuration can produce
+void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* class_mask,
+                    unsigned* fp8_mask);
+// k_pcg.hip
+int mpc_launch_gemv(hipStream_t st, const float* A, int64_t lda, int64_t n, int nrhs,
+                    const double* X, int64_t ldx, double* Y, int64_t ldy, double* part);
+int mpc_launch_trsv_step_fwd(hipStream_t st, const float* L, int64_t ldl, int64_t n, int nb,
+                             int j, const float* dinv, double* W, int64_t ldw, int nrhs,
+                             double* Y, int64_t ldy);
+int mpc_launch_trsv_step_bwd(hipStream_t st, const float* L, int64_t ldl, int64_t n, int nb,
+                             int j, const float* dinv, double* W, int64_t ldw, int nrhs,
+                             double* Xo, int64_t ldxo);
+int mpc_launch_dot(hipStream_t st, int64_t n, int nrhs, const double* a, int64_t lda_,
+                   const double* b, int64_t ldb, double* out, double* part);
+int mpc_launch_pcg_update_xr(hipStream_t st, int64_t n, int nrhs, double* X, int64_t ldx,
+                             double* R, int64_t ldr, const double* P, int64_t ldp,
+                             const double* Q, int64_t ldq, const double* rz, const double* pq,
+                             const int* active);
+int mpc_launch_pcg_update_p(hipStream_t st, int64_t n, int nrhs, double* P, int64_t ldp,
+                            const double* Z, int64_t ldz, const double* rz_new,
+                            const double* rz_old, const int* active);
+int mpc_launch_axpby(hipStream_t st, int64_t n, int nrhs, double a, const double* X,
+                     int64_t ldx, double b, double* Y, int64_t ldy);
+int mpc_launch_copy2d(hipStream_t st, int64_t n, int nrhs, const double* X, int64_t ldx,
+                      double* Y, int64_t ldy);
+int mpc_launch_copy_f32(hipStream_t st, int64_t rows, int64_t cols, const float* A,
+                        int64_t lda, float* B, int64_t ldb);
+// k_matgen.hip
+int mpc_launch_hh_matrix(hipStream_t st, int64_t n, const double* lam, const double* W,
+                         const double* P, const double* g, float* A, int64_t lda);
+int mpc_launch_points(hipStream_t st, float* X, int64_t n, int d, int64_t ldx, int kind,
+                      double p_zero, uint64_t seed);
+int mpc_launch_rbf(hipStream_t st, const float* X, int64_t n, int d, int64_t ldx,
+                   const double* dls, int n_ls, double variance, double nugget, float* K,
+                   int64_t ldk);
+int mpc_launch_convert_f8(hipStream_t st, const float* x, uint8_t* y, int64_t count, int prec);
+int mpc_launch_dinv_from_L(hipStream_t st, const float* L, int64_t ldl, int64_t n, float* dinv);
+int mpc_gemv_nsplit(int64_t n);
+
+// host-side generators shared with mpc_matgen (RNG identical to the device one)
+void mpc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double mpc_normal_host(uint64_t seed, uint64_t stream, uint64_t idx);
+double mpc_uniform_host(uint64_t seed, uint64_t stream, uint64_t idx);

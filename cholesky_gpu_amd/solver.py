@@ -1,0 +1,230 @@
+"""Host-side API over the C ABI, with torch tensors as device-memory plumbing.
+
+Layout convention: a matrix argument is a CUDA (ROCm) torch tensor whose
+memory holds the column-major matrix, i.e. for an n x n matrix M the tensor
+T has shape (n, n) and T[j, i] == M[i, j] (T is M's transpose in torch's
+row-major view).  Symmetric matrices are unaffected.  A block of nrhs
+right-hand sides (column-major n x nrhs) is a contiguous tensor of shape
+(nrhs, n).  Every call synchronises torch's device before handing pointers to
+the library and the library's stream before returning, so results are ready
+for torch ops.
+"""
+import ctypes as C
+
+import torch
+
+from . import _lib as L
+from ._lib import check
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev(t):
+    if not (t.is_cuda and t.is_contiguous()):
+        raise ValueError("expected a contiguous device tensor")
+    return t
+
+
+class Handle:
+    """Owns an mpc_handle (its stream and workspaces) on one device."""
+
+    def __init__(self, device=0):
+        self.lib = L.load()
+        self.device = device
+        h = C.c_void_p()
+        torch.cuda.synchronize(device)
+        check(self.lib.mpc_handle_create(C.byref(h), device), "mpc_handle_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.mpc_handle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(self.lib.mpc_synchronize(self.h), "mpc_synchronize")
+
+    def stream_ptr(self):
+        return self.lib.mpc_get_stream(self.h)
+
+
+def _pre(h):
+    torch.cuda.synchronize(h.device)
+
+
+# ---------------------------------------------------------------- options
+def factor_opts(nb=256, update_prec=L.PREC_SWITCH, ladder=(L.PREC_E4M3, L.PREC_FP32),
+                eps_prime=0.25, fallback=1, profile=0):
+    o = L.FactorOpts()
+    L.load().mpc_factor_opts_default(C.byref(o))
+    o.nb = nb
+    o.update_prec = update_prec
+    for i in range(4):
+        o.ladder[i] = ladder[i] if i < len(ladder) else 0
+    o.n_ladder = len(ladder)
+    o.eps_prime = eps_prime
+    o.fallback = fallback
+    o.profile = profile
+    return o
+
+
+def pcg_opts(tol=1e-10, maxit=500, stop_rule=0, x0_mode=0):
+    o = L.PcgOpts()
+    o.tol, o.maxit, o.stop_rule, o.x0_mode = tol, maxit, stop_rule, x0_mode
+    return o
+
+
+def info_dict(fi):
+    if isinstance(fi, L.FactorInfo):
+        return {"info": fi.info, "retries": fi.retries, "eps_prime_used": fi.eps_prime_used,
+                "min_pivot": fi.min_pivot, "flops_total": fi.flops_total,
+                "syrk_flops": {L.PREC_NAMES[k]: fi.syrk_flops[k] for k in range(7) if fi.syrk_flops[k]},
+                "tiles": {L.PREC_NAMES[k]: fi.tiles[k] for k in range(7) if fi.tiles[k]},
+                "ms_total": fi.ms_total, "ms_potrf": fi.ms_potrf, "ms_trsm": fi.ms_trsm,
+                "ms_quant": fi.ms_quant, "ms_syrk": fi.ms_syrk}
+    return {"iters": fi.iters, "converged": fi.converged, "relres": fi.relres,
+            "kappa_est": fi.kappa_est, "ms_total": fi.ms_total}
+
+
+# ---------------------------------------------------------------- generators
+def matgen_householder(h, n, cond=1e6, lam_max=1e6, seed=20240531, dist=L.DIST_LOGSPACED,
+                       n_reflectors=4, n_cluster=0, cluster_lo=0.0, cluster_hi=0.0, out=None):
+    """A = H Lambda H^T (FP32, n x n) and its exact spectrum (FP64)."""
+    o = L.MatgenOpts()
+    h.lib.mpc_matgen_opts_default(C.byref(o))
+    o.dist, o.cond, o.lam_max, o.n_reflectors, o.seed = dist, cond, lam_max, n_reflectors, seed
+    o.n_cluster, o.cluster_lo, o.cluster_hi = n_cluster, cluster_lo, cluster_hi
+    A = out if out is not None else torch.empty(n, n, dtype=torch.float32, device=h.device)
+    lam = torch.empty(n, dtype=torch.float64, device=h.device)
+    _pre(h)
+    check(h.lib.mpc_matgen(h.h, _ptr(_dev(A)), n, n, C.byref(o), _ptr(lam)), "mpc_matgen")
+    return A, lam
+
+
+def rbf_points(h, n, d, kind=0, p_zero=0.8, seed=1):
+    X = torch.empty(n, d, dtype=torch.float32, device=h.device)
+    _pre(h)
+    check(h.lib.mpc_rbf_points(h.h, _ptr(X), n, d, d, kind, p_zero, seed), "mpc_rbf_points")
+    h.sync()
+    return X
+
+
+def rbf_assemble(h, X, ls, variance=1.0, nugget=0.0, out=None):
+    n, d = X.shape
+    ls = [float(v) for v in (ls if hasattr(ls, "__len__") else [ls])]
+    arr = (C.c_double * len(ls))(*ls)
+    K = out if out is not None else torch.empty(n, n, dtype=torch.float32, device=h.device)
+    _pre(h)
+    check(h.lib.mpc_rbf_assemble(h.h, _ptr(_dev(X)), n, d, d, arr, len(ls), variance, nugget,
+                                 _ptr(_dev(K)), n), "mpc_rbf_assemble")
+    return K
+
+
+def median_pairwise_distance(h, X, nsub=1024, seed=3):
+    n, d = X.shape
+    out = C.c_double(0)
+    _pre(h)
+    check(h.lib.mpc_median_pairwise_distance(h.h, _ptr(_dev(X)), n, d, d, min(nsub, n), seed,
+                                             C.byref(out)), "mpc_median_pairwise_distance")
+    return out.value
+
+
+# ---------------------------------------------------------------- factor / solve
+def factor(h, A, opts=None, want_map=True, want_diag=False):
+    """In-place mixed-precision Cholesky of A (lower).  Returns
+    (info, FactorInfo, prec_map or None, updated_diag or None).  prec_map is
+    an (nt, nt) uint8 tensor with prec_map[j, i] = precision of tile L_ij."""
+    n = A.shape[0]
+    opts = opts or factor_opts()
+    nt = (n + opts.nb - 1) // opts.nb
+    mp = torch.empty(nt, nt, dtype=torch.uint8, device=h.device) if want_map else None
+    dg = torch.empty(n, dtype=torch.float32, device=h.device) if want_diag else None
+    fi = L.FactorInfo()
+    _pre(h)
+    rc = h.lib.mpc_factor(h.h, _ptr(_dev(A)), n, n, C.byref(opts), _ptr(mp), _ptr(dg), C.byref(fi))
+    if rc < 0:
+        check(rc, "mpc_factor")
+    return rc, fi, mp, dg
+
+
+def pcg(h, A, Lf, B, nb=256, opts=None, X=None):
+    """Solve A X = B with the factor Lf as preconditioner.  B: (nrhs, n) float64."""
+    n = A.shape[0]
+    B = B.reshape(-1, n).contiguous()
+    nrhs = B.shape[0]
+    X = X if X is not None else torch.zeros_like(B)
+    pi = L.PcgInfo()
+    opts = opts or pcg_opts()
+    _pre(h)
+    rc = h.lib.mpc_pcg(h.h, _ptr(_dev(A)), n, _ptr(_dev(Lf)), n, n, nb, nrhs, _ptr(_dev(B)), n,
+                       _ptr(_dev(X)), n, C.byref(opts), C.byref(pi))
+    check(rc, "mpc_pcg", allow=(L.ERR_NOT_CONVERGED,))
+    return X, pi, rc
+
+
+def precond_cg(h, A, B, fopts=None, popts=None, X=None):
+    """Factor a workspace copy of A, then PCG.  Returns (X, FactorInfo, PcgInfo, rc)."""
+    n = A.shape[0]
+    B = B.reshape(-1, n).contiguous()
+    nrhs = B.shape[0]
+    X = X if X is not None else torch.zeros_like(B)
+    fi, pi = L.FactorInfo(), L.PcgInfo()
+    fopts = fopts or factor_opts()
+    popts = popts or pcg_opts()
+    _pre(h)
+    rc = h.lib.mpc_precond_cg(h.h, _ptr(_dev(A)), n, n, nrhs, _ptr(_dev(B)), n, _ptr(_dev(X)), n,
+                              C.byref(fopts), C.byref(popts), C.byref(fi), C.byref(pi))
+    if rc < 0:
+        check(rc, "mpc_precond_cg", allow=(L.ERR_NOT_CONVERGED,))
+    return X, fi, pi, rc
+
+
+# ---------------------------------------------------------------- building blocks
+def quantize_panel(h, P, prec_of_tile, tile_rows):
+    """P: (k, m) tensor holding the column-major m x k panel.  Returns
+    (payload (m, 4k) uint8, scales (m, k/32) uint8)."""
+    k, m = P.shape
+    pt = torch.as_tensor(list(prec_of_tile), dtype=torch.uint8).to(h.device)
+    pay = torch.zeros(m + 128, 4 * k, dtype=torch.uint8, device=h.device)
+    sc = torch.zeros(m + 128, k // 32, dtype=torch.uint8, device=h.device)
+    _pre(h)
+    check(h.lib.mpc_quantize_panel(h.h, _ptr(_dev(P)), m, k, m, tile_rows, _ptr(pt), _ptr(pay),
+                                   4 * k, _ptr(sc)), "mpc_quantize_panel")
+    return pay[:m], sc[:m], pt
+
+
+def syrk(h, Cm, payload, scales, k, tile_rows, prec_of_tile):
+    """Cm (m, m) column-major, lower part updated in place: C -= Q Q^T."""
+    m = Cm.shape[0]
+    pt = prec_of_tile if torch.is_tensor(prec_of_tile) else torch.as_tensor(
+        list(prec_of_tile), dtype=torch.uint8).to(h.device)
+    _pre(h)
+    check(h.lib.mpc_syrk(h.h, _ptr(_dev(Cm)), m, m, k, _ptr(payload), payload.stride(0),
+                         _ptr(scales), tile_rows, _ptr(pt)), "mpc_syrk")
+    h.sync()
+
+
+def potrf_block(h, A):
+    n = A.shape[0]
+    info = C.c_int(0)
+    _pre(h)
+    rc = h.lib.mpc_potrf_block(h.h, _ptr(_dev(A)), n, n, C.byref(info))
+    if rc < 0:
+        check(rc, "mpc_potrf_block")
+    return info.value
+
+
+def convert_f8(h, x, prec):
+    y = torch.empty(x.numel(), dtype=torch.uint8, device=h.device)
+    _pre(h)
+    check(h.lib.mpc_convert_f8(h.h, _ptr(_dev(x)), _ptr(y), x.numel(), prec), "mpc_convert_f8")
+    h.sync()
+    return y

@@ -1,0 +1,205 @@
+/*
+ * mpchol.h -- C ABI of libmpchol.so: mixed-precision blocked Cholesky used as
+ * a preconditioner for conjugate gradients, on one MI355X (gfx950).
+ *
+ * Drop-in boundary for the reference's driver API (paths relative to the
+ * reference repository):
+ *   mpc_matgen         <- generatePSD            src/matgen.cuh:188-194
+ *                         (DistType ordinals     src/kernels_macros.cuh:24-29)
+ *   mpc_factor         <- switching_chol         src/switching_chol.cuh:19-22
+ *   mpc_pcg            <- PCG loop of precond_CG src/mixed_cholesky.cu:145-217
+ *   mpc_precond_cg     <- precond_CG             src/mixed_cholesky.cu:76-78
+ *   mpc_quantize_panel <- convert_fp32_to_mxe4m3 src/fp8_utils.cuh:27-66 and
+ *                         convertFloatToHalf     src/kernels_macros.cuh:115-124
+ *   mpc_syrk           <- LtMxfp8Matmul / LtHSgemm_fp16in_fp32out / LtSgemm
+ *                         src/fp8_utils.cuh:136,304,231 and the WMMA SYRK
+ *                         src/syrk_gemm.cuh:14-124
+ *   mpc_potrf_block    <- cusolverDnSpotrf call  src/switching_chol.cuh:86-95
+ *                         (CPU form micro_cholesky src/micro_chol.hpp:14-44)
+ *   mpc_update_diag    <- update_diag            src/kernels_macros.cuh:177-190
+ *   mpc_rbf_* / mpc_gp_* : no reference counterpart (the RBF/GP workloads and
+ *                         the backward pass are specified by the workload
+ *                         definitions only; src/RBF.cu holds no RBF code).
+ *
+ * Conventions: matrices are column-major with an explicit leading dimension;
+ * every pointer named d* is device memory owned by the caller.  All calls are
+ * asynchronous on the handle's stream except those that return host values,
+ * which synchronise once.  Return value: 0 on success; > 0 LAPACK-style info
+ * (1-based column of the first non-positive pivot); < 0 an MPC_ERR_* code.
+ * A handle is not thread-safe: use one handle per host thread.
+ */
+#ifndef MPCHOL_H
+#define MPCHOL_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPC_VERSION_MAJOR 0
+#define MPC_VERSION_MINOR 1
+
+/* ---- status codes (negative) ------------------------------------------ */
+#define MPC_ERR_ARG            (-1)    /* an argument is invalid (see last error) */
+#define MPC_ERR_HIP            (-100)  /* HIP runtime failure */
+#define MPC_ERR_NOT_CONVERGED  (-101)  /* PCG hit maxit */
+#define MPC_ERR_NAN            (-102)  /* NaN/Inf in the matrix or a panel */
+#define MPC_ERR_NOMEM          (-103)  /* device allocation failed */
+#define MPC_ERR_UNSUPPORTED    (-104)  /* option combination not supported */
+#define MPC_ERR_NO_DEVICE      (-105)
+
+/* ---- precision codes (storage / compute format of a trailing-update tile) */
+#define MPC_PREC_FP32    0
+#define MPC_PREC_BF16    1
+#define MPC_PREC_FP16    2
+#define MPC_PREC_E5M2    3   /* OCP E5M2, one power-of-two scale per tile      */
+#define MPC_PREC_E4M3    4   /* OCP E4M3, one power-of-two scale per tile      */
+#define MPC_PREC_MXE4M3  5   /* OCP MX: E4M3 + E8M0 scale per 32 along k       */
+#define MPC_PREC_MXE5M2  6   /* OCP MX: E5M2 + E8M0 scale per 32 along k       */
+#define MPC_PREC_SWITCH  (-1)/* per-tile switching over opts.ladder           */
+
+/* ---- spectrum distributions: 0..3 keep the reference's DistType ordinals */
+#define MPC_DIST_RANDOM_LOG_UNIFORM 0
+#define MPC_DIST_CLUSTERED          1
+#define MPC_DIST_ARITHMETIC         2
+#define MPC_DIST_GEOMETRIC          3
+#define MPC_DIST_LOGSPACED          4   /* lambda_i increasing, log-spaced */
+
+/* ---- matrix kinds for mpc_matgen */
+#define MPC_MAT_HOUSEHOLDER_SPD 0       /* A = H Lambda H^T, H = prod of reflectors */
+
+typedef struct mpc_handle mpc_handle;
+
+/* ---- handle --------------------------------------------------------------- */
+int         mpc_handle_create(mpc_handle** h, int device);
+int         mpc_handle_destroy(mpc_handle* h);
+/* use a caller-owned hipStream_t (never destroyed by the library); NULL = own */
+int         mpc_set_stream(mpc_handle* h, void* hip_stream);
+void*       mpc_get_stream(mpc_handle* h);
+int         mpc_synchronize(mpc_handle* h);
+const char* mpc_last_error_string(void);
+const char* mpc_version_string(void);
+
+/* ---- matrix generation (generatePSD, matgen.cuh:188-298) ----------------- */
+typedef struct {
+    int      kind;          /* MPC_MAT_HOUSEHOLDER_SPD */
+    int      dist;          /* MPC_DIST_* for the main part of the spectrum */
+    double   cond;          /* ratio of the main part: lambda in [lam_max/cond, lam_max] */
+    double   lam_max;
+    int      n_reflectors;  /* 1..4 */
+    uint64_t seed;
+    int64_t  n_cluster;     /* leading eigenvalues log-spaced in [cluster_lo, cluster_hi] */
+    double   cluster_lo;
+    double   cluster_hi;
+} mpc_matgen_opts;
+void mpc_matgen_opts_default(mpc_matgen_opts* o);
+/* Writes A (n x n, FP32, both triangles).  d_lambda (optional, device, n
+   doubles) receives the exact spectrum. */
+int mpc_matgen(mpc_handle* h, float* dA, int64_t n, int64_t lda,
+               const mpc_matgen_opts* o, double* d_lambda);
+
+/* ---- RBF / GP assembly (no reference counterpart) ----------------------- */
+/* points, row-major n x d (ld >= d): kind 0 = N(0, I_d); kind 1 = each
+   coordinate 0 with probability p_zero, else uniform in (0, 1]            */
+int mpc_rbf_points(mpc_handle* h, float* dX, int64_t n, int d, int64_t ldx,
+                   int kind, double p_zero, uint64_t seed);
+/* K_ij = variance * exp(-1/2 sum_c ((x_ic - x_jc)/ls_c)^2) + nugget [i==j];
+   n_ls = 1 (isotropic) or d (ARD); ls is a host array */
+int mpc_rbf_assemble(mpc_handle* h, const float* dX, int64_t n, int d, int64_t ldx,
+                     const double* ls, int n_ls, double variance, double nugget,
+                     float* dK, int64_t ldk);
+/* median pairwise Euclidean distance of nsub points picked without
+   replacement by a seeded permutation (host result) */
+int mpc_median_pairwise_distance(mpc_handle* h, const float* dX, int64_t n, int d,
+                                 int64_t ldx, int64_t nsub, uint64_t seed, double* out);
+
+/* ---- factorisation (switching_chol, switching_chol.cuh:19-227) --------- */
+typedef struct {
+    int    nb;              /* block size: 64, 128, 256 (default) or 512 */
+    int    update_prec;     /* MPC_PREC_* for every tile, or MPC_PREC_SWITCH */
+    int    ladder[4];       /* switching ladder, lowest precision first */
+    int    n_ladder;
+    double eps_prime;       /* switching threshold (rule below) */
+    int    fallback;        /* 1: on a non-positive pivot restore A, divide
+                               eps_prime by 16 and retry (twice), then all-FP32 */
+    int    profile;         /* 1: time the phases with events (adds syncs) */
+} mpc_factor_opts;
+/* Rule (per panel tile L_ij, i > j): the lowest-precision ladder entry p with
+       u_p * max_{r in tile} sum_k L_rk^2  <=  eps_prime * min_k L_kk^2,
+   the minimum taken over the diagonal blocks factored so far.  A tile pair
+   (i, l) of the trailing update is computed in the higher of the two. */
+
+typedef struct {
+    int    info;            /* 0, or 1-based column of the first bad pivot */
+    int    retries;         /* fallback restarts used */
+    double eps_prime_used;
+    double min_pivot;       /* min L_kk^2 */
+    double flops_total;     /* n^3/3 + n^2/2 + n/6 */
+    double syrk_flops[8];   /* trailing-update flops by compute class (index = prec code) */
+    int64_t tiles[8];       /* panel tiles by storage precision */
+    double ms_total;        /* wall time of the call (device, events) */
+    double ms_potrf, ms_trsm, ms_quant, ms_syrk;   /* profile = 1 only */
+} mpc_factor_info;
+void mpc_factor_opts_default(mpc_factor_opts* o);
+/* In place, lower, column-major (upper triangle untouched).  d_prec_map
+   (optional, device, nt*nt bytes, nt = ceil(n/nb)) receives the precision of
+   each panel tile L_ij at [i + j*nt] (diagonal tiles: FP32; upper: 255).
+   d_updated_diag (optional, device, n floats) receives the Schur-diagonal
+   tracker of update_diag.  info may be NULL. */
+int mpc_factor(mpc_handle* h, float* dA, int64_t n, int64_t lda, const mpc_factor_opts* o,
+               uint8_t* d_prec_map, float* d_updated_diag, mpc_factor_info* info);
+
+/* ---- PCG (precond_CG loop, mixed_cholesky.cu:145-217) ------------------ */
+typedef struct {
+    double tol;             /* target ||b - A x||_2 / ||b||_2 (FP64) */
+    int    maxit;
+    int    stop_rule;       /* 0: true residual (default); 1: reference sqrt(r.z)/||b|| */
+    int    x0_mode;         /* 0: x0 = M^-1 b (reference); 1: x0 = 0; 2: x0 from dX */
+} mpc_pcg_opts;
+typedef struct {
+    int    iters;           /* max over right-hand sides */
+    int    converged;       /* number of converged right-hand sides */
+    double relres;          /* max true relative residual */
+    double kappa_est;       /* Lanczos estimate of cond(M^-1 A) (first rhs) */
+    double ms_total;
+} mpc_pcg_info;
+void mpc_pcg_opts_default(mpc_pcg_opts* o);
+/* Solves A X = B for nrhs columns.  A: FP32 n x n (full storage, only read);
+   L: FP32 lower factor from mpc_factor with block size nb; B, X: FP64 n x nrhs.
+   Returns 0 if all columns converged, MPC_ERR_NOT_CONVERGED otherwise. */
+int mpc_pcg(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_t ldl,
+            int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
+            int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info);
+
+/* ---- factor + solve (precond_CG, mixed_cholesky.cu:76-229) ------------- */
+/* Factors a workspace copy of A (A itself is not modified), then runs PCG.
+   Returns 0 (converged), MPC_ERR_NOT_CONVERGED, or factor info > 0. */
+int mpc_precond_cg(mpc_handle* h, const float* dA, int64_t lda, int64_t n, int nrhs,
+                   const double* dB, int64_t ldb, double* dX, int64_t ldx,
+                   const mpc_factor_opts* fo, const mpc_pcg_opts* po,
+                   mpc_factor_info* finfo, mpc_pcg_info* pinfo);
+
+/* ---- building blocks, exported for tests and advanced callers --------- */
+/* Quantise an m x k panel (column-major, ld) in row tiles of tile_rows rows;
+   prec_of_tile is a device array of ceil(m/tile_rows) codes.  Payload is
+   row-major k-contiguous with row_stride bytes per row; scales is m x (k/32)
+   E8M0 bytes.  k must be a multiple of 32. */
+int mpc_quantize_panel(mpc_handle* h, const float* dP, int64_t m, int64_t k, int64_t ld,
+                       int tile_rows, const uint8_t* d_prec_of_tile, uint8_t* d_payload,
+                       int64_t row_stride, uint8_t* d_scales);
+/* C (m x m lower, FP32, ldc) -= Q Q^T for a quantised panel from
+   mpc_quantize_panel (k multiple of 64, tile_rows multiple of 128). */
+int mpc_syrk(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_t k,
+             const uint8_t* d_payload, int64_t row_stride, const uint8_t* d_scales,
+             int tile_rows, const uint8_t* d_prec_of_tile);
+/* POTRF of one diagonal block (n <= 256) in place; host info result */
+int mpc_potrf_block(mpc_handle* h, float* dA, int n, int64_t lda, int* info);
+/* element-wise FP32 -> 8-bit conversion of a device array (prec E4M3 or E5M2),
+   used for the exhaustive conversion sweep */
+int mpc_convert_f8(mpc_handle* h, const float* dx, uint8_t* dy, int64_t count, int prec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,291 @@
+"""GPU parity tests: libmpchol's kernels (through the C ABI) against the oracle.
+
+Run on an MI355X: python -u -m pytest tests -m gpu -x -v --timeout 120 --timeout-method thread
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from cholesky_gpu_amd import _lib as L  # noqa: E402
+from cholesky_gpu_amd import solver as S  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def to_dev_cm(A):
+    """numpy matrix -> device tensor holding it column-major (tensor = A^T)."""
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(A, np.float32).T)).cuda()
+
+
+def from_dev_cm(T):
+    return T.cpu().numpy().T
+
+
+def spd(n, seed, cond=1e3):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.logspace(0, np.log10(cond), n)
+    A = (Q * lam) @ Q.T
+    return ((A + A.T) / 2).astype(np.float32)
+
+
+# ---------------------------------------------------------------- conversions
+@pytest.mark.parametrize("fmt", [L.PREC_E4M3, L.PREC_E5M2])
+def test_convert_f8_exhaustive(handle, oracle, fmt):
+    """Hardware OCP conversion == lo_float restatement on every float whose
+    magnitude is within the quantiser's domain (|x| <= max finite)."""
+    maxabs = 448.0 if fmt == L.PREC_E4M3 else 57344.0
+    chunk = 1 << 28
+    bad = other = 0
+    for c in range(16):
+        lo = c * chunk
+        say("convert", L.PREC_NAMES[fmt], "chunk", c)
+        bits = torch.arange(lo, lo + chunk, dtype=torch.int64, device="cuda").to(torch.int32)
+        y = S.convert_f8(handle, bits.view(torch.float32), fmt).cpu().numpy()
+        del bits
+        b, o = oracle.sweep(fmt, lo, lo + chunk, y, maxabs)
+        bad += b
+        other += o
+    print("out-of-domain encodings that differ (NaN/overflow):", other)
+    assert bad == 0
+
+
+# ---------------------------------------------------------------- POTRF
+@pytest.mark.parametrize("n", [256, 200, 64, 33])
+def test_potrf_block(handle, oracle, n):
+    say("potrf_block n =", n)
+    A = spd(n, n)
+    T = to_dev_cm(A)
+    info = S.potrf_block(handle, T)
+    assert info == 0
+    Lg = np.tril(from_dev_cm(T))
+    Lo, info_o = oracle.potrf_f32_micro(A)
+    assert info_o == 0
+    Lo = np.tril(Lo)
+    rel = np.linalg.norm(Lg - Lo) / np.linalg.norm(Lo)
+    assert rel < 2e-5, rel
+    bwd = np.linalg.norm(Lg.astype(np.float64) @ Lg.T.astype(np.float64) - A) / np.linalg.norm(A)
+    assert bwd < 4 * n * 6e-8, bwd
+    # strict upper triangle untouched
+    assert np.array_equal(np.triu(from_dev_cm(T), 1), np.triu(A, 1))
+
+
+def test_potrf_block_indefinite(handle):
+    say("potrf_block indefinite")
+    A = np.eye(64, dtype=np.float32)
+    A[40, 40] = -1.0
+    info = S.potrf_block(handle, to_dev_cm(A))
+    assert info == 41
+
+
+# ---------------------------------------------------------------- quantiser
+ALL_FORMATS = [L.PREC_E4M3, L.PREC_E5M2, L.PREC_MXE4M3, L.PREC_MXE5M2, L.PREC_BF16,
+               L.PREC_FP16, L.PREC_FP32]
+
+
+@pytest.mark.parametrize("m,k,tile_rows", [(7 * 128 + 37, 256, 128), (512, 128, 256), (300, 64, 128)])
+def test_quantize_panel_bitexact(handle, oracle, m, k, tile_rows):
+    say("quantize", m, k, tile_rows)
+    rng = np.random.default_rng(m + k)
+    P = (rng.standard_normal((m, k)) * np.exp(rng.uniform(-8, 8, (m, 1)))).astype(np.float32)
+    P[3, 5] = 0.0
+    P[4, 7] = -0.0
+    P[5, :] = 1e-30   # tiny row: subnormal outputs
+    nt = (m + tile_rows - 1) // tile_rows
+    precs = [ALL_FORMATS[t % len(ALL_FORMATS)] for t in range(nt)]
+    pay_g, sc_g, _ = S.quantize_panel(handle, to_dev_cm(P), precs, tile_rows)
+    pay_o, sc_o, _ = oracle.quantize_panel(P, precs, tile_rows)
+    pay_g = pay_g.cpu().numpy()
+    sc_g = sc_g.cpu().numpy()
+    for t in range(nt):
+        r0, r1 = t * tile_rows, min(m, (t + 1) * tile_rows)
+        nbytes = k * {L.PREC_FP32: 4, L.PREC_BF16: 2, L.PREC_FP16: 2}.get(precs[t], 1)
+        assert np.array_equal(pay_g[r0:r1, :nbytes], pay_o[r0:r1, :nbytes]), L.PREC_NAMES[precs[t]]
+        assert np.array_equal(sc_g[r0:r1], sc_o[r0:r1]), L.PREC_NAMES[precs[t]]
+
+
+# ---------------------------------------------------------------- SYRK
+SYRK_CASES = [
+    (L.PREC_E4M3, L.PREC_E4M3), (L.PREC_MXE4M3, L.PREC_MXE4M3), (L.PREC_E5M2, L.PREC_E5M2),
+    (L.PREC_MXE5M2, L.PREC_E4M3), (L.PREC_BF16, L.PREC_BF16), (L.PREC_BF16, L.PREC_E4M3),
+    (L.PREC_FP16, L.PREC_FP16), (L.PREC_FP32, L.PREC_FP32), (L.PREC_FP32, L.PREC_MXE4M3),
+    (L.PREC_FP32, L.PREC_BF16),
+]
+
+
+@pytest.mark.parametrize("pa,pb", SYRK_CASES)
+def test_syrk_classes(handle, oracle, pa, pb):
+    """C -= Q Q^T with the tiles of Q in formats (pa, pb, pa, ...): exact
+    products of the dequantised operands, FP32 accumulation."""
+    m, k, tr = 3 * 128 + 70, 256, 128
+    say("syrk", L.PREC_NAMES[pa], L.PREC_NAMES[pb], m, k)
+    rng = np.random.default_rng(pa * 10 + pb)
+    P = (rng.standard_normal((m, k)) * np.exp(rng.uniform(-3, 3, (m, 1)))).astype(np.float32)
+    Cm = rng.standard_normal((m, m)).astype(np.float32)
+    nt = (m + tr - 1) // tr
+    precs = [pa if t % 2 == 0 else pb for t in range(nt)]
+    pay, sc, pt = S.quantize_panel(handle, to_dev_cm(P), precs, tr)
+    _, _, deq = oracle.quantize_panel(P, precs, tr)
+    Cd = to_dev_cm(Cm)
+    S.syrk(handle, Cd, pay, sc, k, tr, pt)
+    got = from_dev_cm(Cd)
+    ref = Cm.astype(np.float64) - deq.astype(np.float64) @ deq.astype(np.float64).T
+    low = np.tril(np.ones((m, m), bool))
+    scale = np.abs(Cm).astype(np.float64) + np.abs(deq).astype(np.float64) @ np.abs(deq).astype(np.float64).T
+    err = np.abs(got - ref)[low] / scale[low]
+    assert err.max() < 3e-6, err.max()
+    assert np.array_equal(np.triu(got, 1), np.triu(Cm, 1))   # upper untouched
+
+
+def test_syrk_ones_kat(handle, oracle):
+    """syrk_gemm.cuh:252-279 generalised: A = ones(m x k) -> A A^T = k exactly."""
+    m, k = 256, 256
+    say("syrk ones KAT")
+    for prec in [L.PREC_E4M3, L.PREC_BF16, L.PREC_FP16, L.PREC_FP32, L.PREC_MXE5M2]:
+        P = np.ones((m, k), np.float32)
+        pay, sc, pt = S.quantize_panel(handle, to_dev_cm(P), [prec, prec], 128)
+        Cd = to_dev_cm(np.zeros((m, m), np.float32))
+        S.syrk(handle, Cd, pay, sc, k, 128, pt)
+        got = from_dev_cm(Cd)
+        assert np.all(got[np.tril_indices(m)] == -float(k)), L.PREC_NAMES[prec]
+
+
+# ---------------------------------------------------------------- generators
+def test_matgen_householder(handle, oracle):
+    n = 1000
+    say("matgen householder", n)
+    A, lam = S.matgen_householder(handle, n, cond=1e6, lam_max=1e6, seed=7)
+    Ag = from_dev_cm(A)
+    Ao, lam_o = oracle.householder_spd(n, 1e6, 1e6, 7)
+    assert np.array_equal(Ag, Ag.T)
+    assert np.allclose(lam.cpu().numpy(), lam_o, rtol=1e-15, atol=0)
+    rel = np.abs(Ag - Ao).max() / np.abs(Ao).max()
+    assert rel < 1e-6, rel
+    ev = np.linalg.eigvalsh(Ag.astype(np.float64))
+    assert np.allclose(np.sort(ev)[-10:], np.sort(lam_o)[-10:], rtol=1e-6)
+
+
+def test_rbf_points_and_kernel(handle, oracle):
+    say("rbf points + assembly")
+    X1 = S.rbf_points(handle, 500, 20, kind=1, p_zero=0.8, seed=5).cpu().numpy()
+    assert np.array_equal(X1, oracle.points_sparse_image(500, 20, 0.8, 5))
+    X0 = S.rbf_points(handle, 300, 16, kind=0, seed=9).cpu().numpy()
+    assert np.allclose(X0, oracle.points_gaussian(300, 16, 9), rtol=1e-6, atol=1e-6)
+    for ls, nug, X in [(4.0, 1e-2, X0), ([0.5 + 0.1 * i for i in range(16)], 1e-3, X0),
+                       (3.0, 0.0, X1)]:
+        Xd = torch.from_numpy(np.ascontiguousarray(X)).cuda()
+        K = from_dev_cm(S.rbf_assemble(handle, Xd, ls, 1.0, nug))
+        Ko = np.asarray(oracle.rbf_matrix(X, ls, 1.0, nug))
+        assert np.abs(K - Ko).max() < 2e-6
+        assert np.array_equal(K, K.T)
+
+
+# ---------------------------------------------------------------- factor + PCG
+def hh(oracle, n, seed=11):
+    A, lam = oracle.householder_spd(n, 1e6, 1e6, seed)
+    return A
+
+
+@pytest.mark.parametrize("n,nb,prec,ladder", [
+    (1024, 256, L.PREC_FP32, (L.PREC_FP32,)),
+    (1024, 256, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32)),
+    (1100, 128, L.PREC_SWITCH, (L.PREC_MXE4M3, L.PREC_BF16, L.PREC_FP32)),
+    (2048, 256, L.PREC_SWITCH, (L.PREC_E5M2, L.PREC_FP32)),
+    (700, 64, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32)),
+])
+def test_factor_matches_oracle(handle, oracle, n, nb, prec, ladder):
+    say("factor", n, nb, prec, ladder)
+    A = hh(oracle, n)
+    T = to_dev_cm(A)
+    opts = S.factor_opts(nb=nb, update_prec=prec, ladder=ladder, eps_prime=0.25, fallback=0)
+    rc, fi, mp, dg = S.factor(handle, T, opts, want_map=True, want_diag=True)
+    assert rc == 0, rc
+    Lg = np.tril(from_dev_cm(T))
+    gmap = mp.cpu().numpy().T          # gmap[i, j] = precision of tile L_ij
+    # oracle on the GPU's schedule (threshold flips would otherwise differ)
+    Lo, rco, omap, udo, ofi = oracle.factor_mixed(A, nb=nb, update_prec=prec, ladder=ladder,
+                                                  eps_prime=0.25, map_in=gmap,
+                                                  tile_rows=max(nb, 128))
+    assert rco == 0
+    Lo = np.tril(Lo)
+    rel = np.linalg.norm(Lg - Lo) / np.linalg.norm(Lo)
+    assert rel < 5e-4, rel
+    # the oracle's own rule gives (nearly) the same schedule
+    _, _, omap2, _, _ = oracle.factor_mixed(A, nb=nb, update_prec=prec, ladder=ladder,
+                                            eps_prime=0.25, tile_rows=max(nb, 128))
+    low = np.tril(np.ones_like(gmap, bool), -1) & (omap2 != 255)
+    flips = np.sum(gmap[low] != omap2[low])
+    assert flips <= max(1, low.sum() // 50), (flips, low.sum())
+    # Schur-diagonal tracker
+    assert np.allclose(dg.cpu().numpy(), udo, rtol=1e-3, atol=1e-3 * np.abs(np.diag(A)).max())
+    # flop accounting
+    assert abs(sum(fi.syrk_flops) - sum(ofi.flops)) / sum(ofi.flops) < 1e-9
+
+
+@pytest.mark.parametrize("n,nb,ladder,nrhs", [
+    (1024, 256, (L.PREC_FP32,), 1),
+    (1536, 256, (L.PREC_E4M3, L.PREC_FP32), 1),
+    (1024, 128, (L.PREC_MXE4M3, L.PREC_FP32), 3),
+])
+def test_pcg_matches_oracle(handle, oracle, n, nb, ladder, nrhs):
+    say("pcg", n, nb, ladder, nrhs)
+    A = hh(oracle, n, seed=n)
+    rng = np.random.default_rng(n)
+    B = rng.standard_normal((nrhs, n))
+    Ad = to_dev_cm(A)
+    Bd = torch.from_numpy(B).cuda()
+    fo = S.factor_opts(nb=nb, update_prec=L.PREC_SWITCH if len(ladder) > 1 else ladder[0],
+                       ladder=ladder, eps_prime=0.25)
+    po = S.pcg_opts(tol=1e-10, maxit=200)
+    X, fi, pi, rc = S.precond_cg(handle, Ad, Bd, fo, po)
+    assert rc == 0, (rc, S.info_dict(pi))
+    Xh = X.cpu().numpy()
+    A64 = A.astype(np.float64)
+    for j in range(nrhs):
+        rr = np.linalg.norm(B[j] - A64 @ Xh[j]) / np.linalg.norm(B[j])
+        assert rr <= 1e-10, rr
+    # same factor on the CPU: run the oracle PCG on the GPU's L
+    Lg = torch.empty_like(Ad)
+    Lg.copy_(Ad)
+    rc2, fi2, _, _ = S.factor(handle, Lg, fo)
+    assert rc2 == 0
+    Lnp = np.tril(from_dev_cm(Lg))
+    _, it_o, rr_o, kap = oracle.pcg(A, Lnp, B[0], tol=1e-10, maxit=200)
+    assert it_o > 0
+    assert abs(pi.iters - it_o) <= 2, (pi.iters, it_o)
+
+
+def test_factor_breakdown_reports_info(handle):
+    """An indefinite matrix: info = failing column (no fallback), and the
+    fallback path ends with the same info after trying FP32."""
+    say("factor breakdown")
+    n = 512
+    A = np.eye(n, dtype=np.float32) * 4
+    A[300, 300] = -2.0
+    T = to_dev_cm(A)
+    rc, fi, _, _ = S.factor(handle, T, S.factor_opts(nb=256, fallback=0))
+    assert rc == 301
+    T = to_dev_cm(A)
+    rc, fi, _, _ = S.factor(handle, T, S.factor_opts(nb=256, fallback=1))
+    assert rc == 301 and fi.retries >= 1
+
+
+def test_reference_api(handle, oracle):
+    from cholesky_gpu_amd import reference_api as R
+    say("reference api")
+    n = 512
+    dA = torch.empty(n, n, dtype=torch.float64, device="cuda")
+    R.generatePSD(dA, n, 100.0, R.DistType.Arithmetic, h=handle)
+    lam = np.linalg.eigvalsh(dA.cpu().numpy())
+    assert abs(lam.max() / lam.min() - 100.0) / 100.0 < 1e-3
+    b = torch.from_numpy(np.random.default_rng(0).standard_normal(n)).cuda()
+    x = torch.zeros_like(b)
+    rc = R.precond_CG(dA, x, b, n, 128, h=handle, tol=1e-12)
+    assert rc == 0
+    r = b.cpu().numpy() - dA.cpu().numpy() @ x.cpu().numpy()
+    assert np.linalg.norm(r) / np.linalg.norm(b.cpu().numpy()) < 1e-10

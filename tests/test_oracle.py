@@ -1,0 +1,293 @@
+"""CPU tests that pin the oracle (oracle/oracle.c) before it is trusted.
+
+Each check is chosen so that a plausible mistake in the restatement (wrong
+tie rule, wrong subnormal threshold, transposed operand, dropped term) fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+# ---------------------------------------------------------------- 8-bit formats
+# lo_float.h:789-847 (e4m3fn) and :962-1025 (e5m2) special values and limits
+E4M3_TABLE = [
+    (0.0, 0x00), (-0.0, 0x80), (448.0, 0x7E), (-448.0, 0xFE), (464.0, 0x7E),
+    (464.0001, 0x7F), (1e9, 0x7F), (-1e9, 0xFF), (math.inf, 0x7F), (-math.inf, 0xFF),
+    (math.nan, 0x80), (2.0 ** -9, 0x01), (2.0 ** -10, 0x00), (3 * 2.0 ** -11, 0x01),
+    (3 * 2.0 ** -10, 0x02), (2.0 ** -6, 0x08), (1.0, 0x38), (0.125, 0x20), (1.0625, 0x38), (1.1875, 0x3A),
+    (-3.25, 0xC5), (2.0 ** -10 * 1.001, 0x01), (-(2.0 ** -12), 0x80),
+]
+E5M2_TABLE = [
+    (0.0, 0x00), (-0.0, 0x80), (57344.0, 0x7B), (61440.0, 0x7C), (61439.0, 0x7B),
+    (math.inf, 0x7C), (-math.inf, 0xFC), (math.nan, 0x7E), (2.0 ** -16, 0x01),
+    (2.0 ** -17, 0x00), (2.0 ** -14, 0x04), (1.0, 0x3C), (1.125, 0x3C), (1.375, 0x3E),
+]
+
+
+@pytest.mark.parametrize("x,code", E4M3_TABLE)
+def test_e4m3_table(x, code):
+    assert O.lib().orc_f32_to_e4m3(x) == code
+
+
+@pytest.mark.parametrize("x,code", E5M2_TABLE)
+def test_e5m2_table(x, code):
+    assert O.lib().orc_f32_to_e5m2(x) == code
+
+
+@pytest.mark.parametrize("fmt", [O.E4M3, O.E5M2])
+def test_decode_roundtrip(fmt):
+    """Every finite code decodes and re-encodes to itself (lo_float's -0 = 0x80
+    is its NaN code, so 0x80 is excluded for E4M3)."""
+    for v in range(256):
+        x = O.lib().orc_e4m3_to_f32(v) if fmt == O.E4M3 else O.lib().orc_e5m2_to_f32(v)
+        if not math.isfinite(x):
+            continue
+        enc = O.lib().orc_f32_to_e4m3(x) if fmt == O.E4M3 else O.lib().orc_f32_to_e5m2(x)
+        if x == 0.0:
+            assert enc in (0x00, 0x80)
+        else:
+            assert enc == v, (v, x, enc)
+
+
+@pytest.mark.parametrize("fmt", [O.E4M3, O.E5M2])
+def test_rne_against_closed_form_sample(fmt):
+    """Independent frexp/ldexp rounding on 2^22 stratified inputs (the full 2^32
+    sweep is test_rne_against_closed_form_exhaustive)."""
+    rng = np.random.default_rng(fmt)
+    bits = rng.integers(0, 2 ** 32, size=1 << 22, dtype=np.uint64)
+    lo = 0
+    # the C self-check over a contiguous window plus random windows
+    assert O.selfcheck_f8(fmt, 0x3F000000, 0x3F000000 + (1 << 22)) == 0
+    assert O.selfcheck_f8(fmt, 0x00000000, 1 << 22) == 0     # zero and FP32 subnormals
+    for b in bits[:16]:
+        start = int(b) & ~0xFFFF
+        assert O.selfcheck_f8(fmt, start, start + 0x10000) == 0
+    del lo
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("fmt", [O.E4M3, O.E5M2])
+def test_rne_against_closed_form_exhaustive(fmt):
+    """All 2^32 float bit patterns (about 15 s per format on 8 threads)."""
+    assert O.selfcheck_f8(fmt) == 0
+
+
+def test_bf16_f16_against_numpy():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.standard_normal(20000) * np.exp(rng.uniform(-30, 30, 20000)),
+                        [0.0, -0.0, 65504.0, 65520.0, 65519.99, 6e-8, 3e-8, 2.0 ** -24,
+                         2.0 ** -25, 2.0 ** -25 * 1.0001]]).astype(np.float32)
+    f16 = np.array([O.lib().orc_f32_to_f16(float(v)) for v in x], np.uint16)
+    with np.errstate(over="ignore"):
+        expect = x.astype(np.float16).view(np.uint16)
+    assert np.array_equal(f16, expect)
+    # bfloat16 RNE closed form: round(x / ulp) * ulp on the 8-bit mantissa grid
+    for v in x[:2000]:
+        b = O.lib().orc_f32_to_bf16(float(v))
+        back = O.lib().orc_bf16_to_f32(b)
+        if v == 0:
+            assert back == 0
+            continue
+        e = math.floor(math.log2(abs(float(v))))
+        ulp = 2.0 ** (e - 7)
+        q = abs(float(v)) / ulp
+        r = math.floor(q)
+        if q - r > 0.5 or (q - r == 0.5 and r % 2 == 1):
+            r += 1
+        assert abs(back) == r * ulp, (v, back)
+
+
+def test_scale_exp_closed_form():
+    for fmt, mx in [(O.E4M3, 448.0), (O.E5M2, 57344.0), (O.MXE4M3, 448.0)]:
+        for am in [1.0, 448.0, 449.0, 0.001, 3e-39, 1e30, 57344.0, 1.75, 1.7500001]:
+            e = O.scale_exp(am, fmt)
+            assert am / 2.0 ** e <= mx
+            if e > -127:
+                assert am / 2.0 ** (e - 1) > mx      # smallest such power of two
+    assert O.scale_exp(0.0, O.E4M3) == 0
+    assert O.scale_exp(5.0, O.FP32) == 0
+
+
+def test_quantize_panel_values():
+    rng = np.random.default_rng(1)
+    P = (rng.standard_normal((300, 64)) * 10).astype(np.float32)
+    for prec in [O.E4M3, O.E5M2, O.MXE4M3, O.MXE5M2, O.BF16, O.FP16, O.FP32]:
+        pay, sc, deq = O.quantize_panel(P, [prec] * 3, 128)
+        rel = np.abs(deq - P) / np.maximum(np.abs(P), 1e-30)
+        u = O.unit_roundoff(prec)
+        big = np.abs(P) > np.abs(P).max() * 2.0 ** -5    # away from the subnormal range
+        assert rel[big].max() <= u * (1 + 1e-6), O.unit_roundoff(prec)
+        if prec in (O.MXE4M3, O.MXE5M2):
+            # every 32-block of a row has its own power-of-two scale
+            e = sc.astype(int) - 127
+            blk = np.abs(P).reshape(300, 2, 32).max(axis=2)
+            mx = 448.0 if prec == O.MXE4M3 else 57344.0
+            assert np.all(blk / 2.0 ** e <= mx)
+
+
+# ---------------------------------------------------------------- RNG and generators
+def test_philox_known_answers():
+    """Random123 kat_vectors for philox4x32-10."""
+    assert list(O.philox([0, 0, 0, 0], [0, 0])) == [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]
+    assert list(O.philox([0xffffffff] * 4, [0xffffffff] * 2)) == [0x408f276d, 0x41c83b0e,
+                                                                  0xa20bc7c6, 0x6d5451fd]
+    assert list(O.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
+                         [0xa4093822, 0x299f31d0])) == [0xd16cfe09, 0x94fdcceb, 0x5001e420,
+                                                        0x24126ea1]
+
+
+def test_normal_moments():
+    z = np.array([O.normal(5, 0, i) for i in range(40000)])
+    assert abs(z.mean()) < 0.02 and abs(z.std() - 1) < 0.02
+    u = np.array([O.uniform(5, 1, i) for i in range(40000)])
+    assert u.min() >= 0 and u.max() < 1 and abs(u.mean() - 0.5) < 0.01
+
+
+@pytest.mark.parametrize("dist", [0, 1, 2, 3, 4])
+def test_householder_spectrum(dist):
+    """H Lambda H^T has exactly the spectrum Lambda (H orthogonal)."""
+    n = 300
+    cond = 1e4
+    A, lam = O.householder_spd(n, cond, 1.0, 17, dist=dist)
+    assert np.array_equal(A, A.T)
+    Hl = np.zeros((n, n))
+    W, Pm, g = O.hh_setup(n, 4, 17, lam)
+    for i in range(n):
+        for j in range(n):
+            Hl[i, j] = O.lib().orc_hh_entry(i, j, lam.ctypes.data, W.ctypes.data, Pm.ctypes.data,
+                                            g.ctypes.data)
+    ev = np.linalg.eigvalsh(Hl)
+    assert np.allclose(np.sort(ev), np.sort(lam), rtol=1e-9, atol=1e-12 * lam.max())
+    if dist == 4:
+        assert np.isclose(lam.max() / lam.min(), cond)
+
+
+def test_spectrum_cluster():
+    lam = O.spectrum(4, 1000, 1e4, 1.0, 64, 1e-8, 1.1e-8, 0)
+    assert np.all(lam[:64] >= 1e-8 * (1 - 1e-12)) and np.all(lam[:64] <= 1.1e-8 * (1 + 1e-12))
+    assert np.isclose(lam[64:].min(), 1e-4) and np.isclose(lam.max(), 1.0)
+
+
+def test_rbf_closed_forms():
+    from scipy.spatial.distance import cdist
+    X = O.points_gaussian(200, 16, 3)
+    K = O.rbf_matrix(X, 4.0, 1.0, 1e-2)
+    assert np.allclose(np.diag(K), 1.0 + 1e-2)
+    assert np.array_equal(K, K.T)
+    D2 = cdist(X.astype(np.float64), X.astype(np.float64), "sqeuclidean")
+    Kr = np.exp(-D2 / (2 * 16.0)) + 1e-2 * np.eye(200)
+    assert np.abs(K - Kr).max() < 1e-6
+    ls = np.linspace(0.5, 2.0, 16)
+    Ka = O.rbf_matrix(X, ls, 1.0, 0.0)
+    D2a = cdist(X / ls, X / ls, "sqeuclidean")
+    assert np.abs(Ka - np.exp(-0.5 * D2a)).max() < 1e-6
+    Xs = O.points_sparse_image(1000, 50, 0.8, 9)
+    frac0 = np.mean(Xs == 0)
+    assert abs(frac0 - 0.8) < 0.02 and Xs.max() <= 1.0 and Xs.min() >= 0.0
+
+
+# ---------------------------------------------------------------- factor + PCG
+def test_micro_cholesky_matches_reference_build():
+    """orc_potrf_f32_micro vs the reference's own micro_cholesky
+    (micro_chol.hpp:14-44 compiled from the reference tree into oracle/_ref)."""
+    r = O.ref_lib()
+    if r is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(0)
+    for n in [1, 2, 17, 64, 128]:
+        M = rng.standard_normal((n, n))
+        A = (M @ M.T + n * np.eye(n)).astype(np.float32)
+        L1, info = O.potrf_f32_micro(A)
+        L2 = np.array(A, order="F", copy=True)
+        r.ref_micro_cholesky(L2.ctypes.data, n, n)
+        assert info == 0
+        assert np.array_equal(L1.view(np.uint32), L2.view(np.uint32))
+
+
+def test_potrf_closed_forms():
+    L, info = O.potrf_f64(np.array([[4.0, 2.0], [2.0, 3.0]]))
+    assert info == 0 and np.allclose(L, [[2, 0], [1, math.sqrt(2)]])
+    A = np.array([[25.0, 15, -5], [15, 18, 0], [-5, 0, 11]])
+    L, info = O.potrf_f64(A)
+    assert np.allclose(L, [[5, 0, 0], [3, 3, 0], [-1, 1, 3]])
+    _, info = O.potrf_f64(np.array([[1.0, 2.0], [2.0, 1.0]]))
+    assert info == 2
+    Lm, info = O.potrf_f32_micro(np.array([[4.0, 2.0], [2.0, -3.0]], np.float32))
+    assert info == 2
+
+
+def test_potrf_f64_vs_numpy():
+    rng = np.random.default_rng(2)
+    M = rng.standard_normal((200, 200))
+    A = M @ M.T + 200 * np.eye(200)
+    L, info = O.potrf_f64(A)
+    assert info == 0 and np.allclose(L, np.linalg.cholesky(A), rtol=1e-12, atol=1e-12)
+
+
+def test_syrk_ones_kat():
+    """syrk_gemm.cuh:252-279: ones(256 x 256) A A^T = 256 everywhere (exact)."""
+    Cm = np.zeros((256, 256), np.float32, order="F")
+    O.syrk_lower(Cm, np.ones((256, 256), np.float32))
+    assert np.all(Cm[np.tril_indices(256)] == -256.0)
+    assert np.all(np.triu(Cm, 1) == 0.0)
+
+
+def test_factor_fp32_matches_numpy():
+    A, _ = O.householder_spd(512, 1e4, 1.0, 3)
+    L, rc, mp, ud, fi = O.factor_mixed(A, nb=128, update_prec=O.FP32)
+    assert rc == 0
+    Ln = np.linalg.cholesky(A.astype(np.float64))
+    assert np.linalg.norm(np.tril(L) - Ln) / np.linalg.norm(Ln) < 1e-4
+    # the Schur-diagonal tracker (update_diag) ends at a_ii minus the squares of
+    # row i over the panels before i's own block: the diagonal handed to POTRF
+    c0 = (np.arange(512) // 128) * 128
+    expect = np.array([A[i, i] - np.sum(Ln[i, :c0[i]] ** 2) for i in range(512)])
+    assert np.allclose(ud, expect, rtol=1e-3, atol=1e-5)
+    assert np.isclose(fi.min_pivot, (np.diag(Ln) ** 2).min(), rtol=1e-4)
+    # flop accounting: SYRK flops sum_j s(s+1)b over the steps (lower incl. diag)
+    s = np.arange(512 - 128, 0, -128)
+    assert np.isclose(sum(fi.flops), float(np.sum(s * (s + 1) * 128)), rtol=1e-12)
+
+
+def test_factor_mixed_rule_and_parity():
+    A, _ = O.householder_spd(1024, 1e6, 1e6, 11)
+    L, rc, mp, ud, fi = O.factor_mixed(A, nb=128, update_prec=-1, ladder=(O.E4M3, O.FP32),
+                                       eps_prime=0.25)
+    assert rc == 0
+    assert fi.flops[O.E4M3] > 0 and fi.flops[O.FP32] > 0
+    # replaying the schedule reproduces the factor exactly
+    L2, rc2, mp2, _, _ = O.factor_mixed(A, nb=128, update_prec=-1, ladder=(O.E4M3, O.FP32),
+                                        eps_prime=0.25, map_in=mp)
+    assert rc2 == 0 and np.array_equal(mp, mp2) and np.array_equal(L, L2)
+    # the rule itself
+    lad = np.array([O.E4M3, O.FP32], np.int32)
+    assert O.lib().orc_rule_minpivot(lad.ctypes.data, 2, 1.0, 16.0, 1.0) == O.E4M3
+    assert O.lib().orc_rule_minpivot(lad.ctypes.data, 2, 1.0, 16.1, 1.0) == O.FP32
+
+
+def test_pcg_exact_and_iterations():
+    n = 600
+    A, _ = O.householder_spd(n, 1e6, 1e6, 5)
+    b = np.random.default_rng(1).standard_normal(n)
+    L, rc, _, _, _ = O.factor_mixed(A, nb=128, update_prec=O.FP32)
+    x, it, rr, kap = O.pcg(A, np.tril(L), b, tol=1e-10)
+    assert 0 < it <= 6 and rr <= 1e-10
+    xe = np.linalg.solve(A.astype(np.float64), b)
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-4
+    assert 1.0 <= kap < 1.5
+    # identity preconditioner (L = I): plain CG does not reach 1e-6 in 100 iterations
+    x2, it2, rr2, kap2 = O.pcg(A, np.eye(n, dtype=np.float32), b, tol=1e-6, maxit=100)
+    assert it2 == -1 and rr2 > 1e-6
+    # reference stopping rule sqrt(r.z)/||b||
+    _, it3, _, _ = O.pcg(A, np.tril(L), b, tol=1e-10, stop_rule=1)
+    assert it3 > 0
+
+
+def test_factor_breakdown_info():
+    A = np.eye(256, dtype=np.float32)
+    A[100, 100] = -1
+    L, rc, _, _, _ = O.factor_mixed(A, nb=64, update_prec=O.FP32)
+    assert rc == 101
