@@ -25,7 +25,8 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-I", os.path.join(HERE, "..", "include"),
 ]
-SOURCES = ["api.cpp", "k_potrf.hip", "k_panel.hip", "k_syrk.hip", "k_pcg.hip", "k_matgen.hip"]
+SOURCES = ["api.cpp", "k_potrf.hip", "k_panel.hip", "k_syrk.hip", "k_pcg.hip", "k_matgen.hip",
+           "k_tri.hip"]
 
 
 def build(verbose=False):

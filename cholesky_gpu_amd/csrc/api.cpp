@@ -125,7 +125,7 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     hipSetDevice(h->device);
     hipStreamSynchronize(h->stream);
     for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
-                             &h->pcg, &h->scratch})
+                             &h->pcg, &h->scratch, &h->winv})
         ws_free(*w);
     if (h->own_stream) hipStreamDestroy(h->stream);
     delete h;
@@ -621,7 +621,8 @@ static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         flops_from_map(hm.data(), n, nb, &fi);
     }
     if (info) *info = fi;
-    // the dinv blocks now describe this factor
+    // the dinv blocks now describe this factor; big-block inverses are stale
+    h->winv_owner = nullptr;
     h->dinv_owner = dA;
     h->dinv_n = n;
     h->dinv_ld = lda;
@@ -650,23 +651,45 @@ struct pcg_bufs {
     int* active;
 };
 
-// z = L^-T L^-1 r  (all columns)
+// z = L^-T L^-1 r  (all columns), blocked with the big-block inverses
 int apply_prec(mpc_handle* h, const float* L, int64_t ldl, int64_t n, int nb, int nrhs,
                const double* R, double* Z, pcg_bufs& b) {
+    (void)nb;
     hipStream_t st = h->stream;
-    const float* dinv = (const float*)h->dinv.ptr;
     int rc = mpc_launch_copy2d(st, n, nrhs, R, n, b.W, n);
     if (rc) return rc;
-    const int64_t nt = (n + nb - 1) / nb;
-    for (int64_t j = 0; j < nt; ++j)
-        if ((rc = mpc_launch_trsv_step_fwd(st, L, ldl, n, nb, (int)j, dinv, b.W, n, nrhs, b.Y, n)))
-            return rc;
-    for (int64_t j = nt - 1; j >= 0; --j)
-        if ((rc = mpc_launch_trsv_step_bwd(st, L, ldl, n, nb, (int)j, dinv, b.Y, n, nrhs, Z, n)))
-            return rc;
-    return 0;
+    return mpc_apply_block_prec(st, L, ldl, n, h->winv_bo, (const float*)h->winv.ptr, b.W, b.Y, Z,
+                                n, nrhs);
 }
 }  // namespace
+
+// 32x32 diagonal inverses and big-block inverses of L, cached per factor buffer
+static int prepare_prec(mpc_handle* h, const float* dL, int64_t ldl, int64_t n) {
+    hipStream_t st = h->stream;
+    int rc;
+    const int64_t nsb = (n + 31) / 32;
+    if (h->dinv_owner != dL || h->dinv_n != n || h->dinv_ld != ldl) {
+        if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nsb + 8) * 1024 * sizeof(float)))) return rc;
+        if ((rc = mpc_launch_dinv_from_L(st, dL, ldl, n, (float*)h->dinv.ptr))) return rc;
+        h->dinv_owner = dL;
+        h->dinv_n = n;
+        h->dinv_ld = ldl;
+        h->winv_owner = nullptr;
+    }
+    const int Bo = mpc_block_bo(n);
+    if (h->winv_owner != dL || h->winv_n != n || h->winv_ld != ldl || h->winv_bo != Bo) {
+        const int64_t nI = (n + Bo - 1) / Bo;
+        if ((rc = mpc_ws_reserve(h->winv, sizeof(float) * (size_t)nI * Bo * Bo))) return rc;
+        if ((rc = mpc_build_block_inverses(st, dL, ldl, n, Bo, (const float*)h->dinv.ptr,
+                                           (float*)h->winv.ptr)))
+            return rc;
+        h->winv_owner = dL;
+        h->winv_n = n;
+        h->winv_ld = ldl;
+        h->winv_bo = Bo;
+    }
+    return 0;
+}
 
 static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_t ldl,
                     int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
@@ -681,15 +704,7 @@ static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL
     MPC_HIP(hipSetDevice(h->device));
     hipStream_t st = h->stream;
     int rc;
-    // diagonal-block inverses of L
-    const int64_t nsb = (n + 31) / 32;
-    if (h->dinv_owner != dL || h->dinv_n != n || h->dinv_ld != ldl) {
-        if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nsb + 8) * 1024 * sizeof(float)))) return rc;
-        if ((rc = mpc_launch_dinv_from_L(st, dL, ldl, n, (float*)h->dinv.ptr))) return rc;
-        h->dinv_owner = dL;
-        h->dinv_n = n;
-        h->dinv_ld = ldl;
-    }
+    if ((rc = prepare_prec(h, dL, ldl, n))) return rc;
     const int ns = mpc_gemv_nsplit(n);
     const size_t vec = (size_t)n * nrhs;
     const size_t part = std::max<size_t>((size_t)ns * vec, (size_t)nrhs * ((n + 2047) / 2048) + 64);
@@ -923,6 +938,29 @@ extern "C" int mpc_syrk(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_
                            d_prec_of_tile, nullptr);
 }
 
+extern "C" int mpc_apply_precond(mpc_handle* h, const float* dL, int64_t ldl, int64_t n,
+                                 int nrhs, const double* dR, int64_t ldr, double* dZ,
+                                 int64_t ldz) {
+    MPC_CHECK(h && dL && dR && dZ, "apply_precond: NULL argument");
+    MPC_CHECK(n >= 1 && ldl >= n && ldr >= n && ldz >= n && nrhs >= 1, "apply_precond: bad shape");
+    MPC_HIP(hipSetDevice(h->device));
+    int rc = prepare_prec(h, dL, ldl, n);
+    if (rc) return rc;
+    const size_t vec = (size_t)n * nrhs;
+    if ((rc = mpc_ws_reserve(h->scratch, sizeof(double) * 3 * vec))) return rc;
+    double* W = (double*)h->scratch.ptr;
+    double* Y = W + vec;
+    double* Z = Y + vec;
+    hipStream_t st = h->stream;
+    if ((rc = mpc_launch_copy2d(st, n, nrhs, dR, ldr, W, n))) return rc;
+    if ((rc = mpc_apply_block_prec(st, dL, ldl, n, h->winv_bo, (const float*)h->winv.ptr, W, Y, Z,
+                                   n, nrhs)))
+        return rc;
+    if ((rc = mpc_launch_copy2d(st, n, nrhs, Z, n, dZ, ldz))) return rc;
+    MPC_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
 extern "C" int mpc_potrf_block(mpc_handle* h, float* dA, int n, int64_t lda, int* info) {
     MPC_CHECK(h && dA, "potrf_block: NULL argument");
     MPC_CHECK(n >= 1 && n <= 256 && lda >= n, "potrf_block: n must be 1..256");
@@ -936,6 +974,7 @@ extern "C" int mpc_potrf_block(mpc_handle* h, float* dA, int n, int64_t lda, int
     MPC_HIP(hipMemcpyAsync(&hi, &sm->info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     MPC_HIP(hipStreamSynchronize(h->stream));
     h->dinv_owner = nullptr;
+    h->winv_owner = nullptr;
     if (info) *info = hi;
     return hi;
 }

@@ -114,7 +114,7 @@ struct tile_ctx {
     float* C;
     int64_t ldc, m;
     int64_t rowbase, colbase;   // first C row / col of this wave's 64x64 quadrant
-    bool diag;                  // output tile straddles the diagonal
+    bool diag;                  // this quadrant straddles the diagonal (rowbase == colbase)
 };
 
 // Element (a, b, i) of the wave's quadrant sits at wave base + 32-bit offset
@@ -177,16 +177,23 @@ __device__ __forceinline__ void mm_fp8(const operand_src& oc, const operand_src&
         rb[q] = t.rowbase + 32 * q + n;
         if (rb[q] >= t.m) rb[q] = t.m - 1;
     }
+    // Operand layout of the 8-bit 32x32x64 matrix-core op (measured on gfx950):
+    // the scale of lane half h (E8M0 block h of the 64-wide K chunk) multiplies
+    // bytes 0-15 of BOTH lane halves when h = 0 and bytes 16-31 of both when
+    // h = 1.  So lane half h carries K [kc + 16h, +16) in bytes 0-15 and
+    // K [kc + 32 + 16h, +16) in bytes 16-31; K block h then spans exactly the
+    // 32 elements the hardware scales with lane half h's scale byte.
     for (int kc = 0; kc < k; kc += 64) {
         v8i A[2], B[2];
         int sa[2], sb[2];
-        const int koff = kc + 32 * h;
+        const int koff = kc + 32 * h;          // scale block of this lane half
+        const int kl = kc + 16 * h;            // first K of bytes 0-15
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-            const uint8_t* pa = oc.payload + ra[q] * oc.row_stride + koff;
-            const uint8_t* pb = orow.payload + rb[q] * orow.row_stride + koff;
-            uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 16);
-            uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 16);
+            const uint8_t* pa = oc.payload + ra[q] * oc.row_stride + kl;
+            const uint8_t* pb = orow.payload + rb[q] * orow.row_stride + kl;
+            uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 32);
+            uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 32);
             A[q] = v8i{(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
             B[q] = v8i{(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
             sa[q] = oc.scales[ra[q] * oc.kb + (koff >> 5)];
@@ -302,7 +309,7 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
     t.m = m;
     t.rowbase = ti * TILE + 64 * wr;
     t.colbase = tj * TILE + 64 * wc;
-    t.diag = diag;
+    t.diag = diag && wr == wc;     // only the diagonal quadrants need the r >= c mask
     if (t.rowbase >= m || t.colbase >= m) return;
     orow.payload = ocol.payload = payload;
     orow.scales = ocol.scales = scales;

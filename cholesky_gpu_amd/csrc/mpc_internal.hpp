@@ -71,11 +71,16 @@ struct mpc_handle {
     mpc_workspace amat;       // copy of A for precond_cg / fallback
     mpc_workspace pcg;        // PCG vectors
     mpc_workspace scratch;    // generic
+    mpc_workspace winv;       // inverses of the big diagonal blocks of L (Bo x Bo each)
     // provenance of the dinv blocks: the factor they belong to
     const float* dinv_owner = nullptr;
     int64_t      dinv_n = 0;
     int64_t      dinv_ld = 0;
     int          num_cu = 0;
+    // provenance of the big-block inverses
+    const float* winv_owner = nullptr;
+    int64_t      winv_n = 0, winv_ld = 0;
+    int          winv_bo = 0;
 };
 
 // grow-only device allocation in a handle workspace
@@ -155,6 +160,12 @@ int mpc_launch_rbf(hipStream_t st, const float* X, int64_t n, int d, int64_t ldx
 int mpc_launch_convert_f8(hipStream_t st, const float* x, uint8_t* y, int64_t count, int prec);
 int mpc_launch_dinv_from_L(hipStream_t st, const float* L, int64_t ldl, int64_t n, float* dinv);
 int mpc_gemv_nsplit(int64_t n);
+// k_tri.hip
+int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
+                             const float* dinv, float* W);
+int mpc_apply_block_prec(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
+                         const float* Winv, double* W, double* Y, double* Z, int64_t ld, int nrhs);
+inline int mpc_block_bo(int64_t n) { return n > 8192 ? 2048 : (n > 2048 ? 1024 : 256); }
 
 // host-side generators shared with mpc_matgen (RNG identical to the device one)
 void mpc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -212,6 +212,17 @@ def syrk(h, Cm, payload, scales, k, tile_rows, prec_of_tile):
     h.sync()
 
 
+def apply_precond(h, Lf, R):
+    """Z = (L L^T)^{-1} R; R: (nrhs, n) float64, Lf the factor tensor."""
+    n = Lf.shape[0]
+    R = R.reshape(-1, n).contiguous()
+    Z = torch.empty_like(R)
+    _pre(h)
+    check(h.lib.mpc_apply_precond(h.h, _ptr(_dev(Lf)), n, n, R.shape[0], _ptr(R), n, _ptr(Z), n),
+          "mpc_apply_precond")
+    return Z
+
+
 def potrf_block(h, A):
     n = A.shape[0]
     info = C.c_int(0)

@@ -16,7 +16,9 @@
  *                         src/syrk_gemm.cuh:14-124
  *   mpc_potrf_block    <- cusolverDnSpotrf call  src/switching_chol.cuh:86-95
  *                         (CPU form micro_cholesky src/micro_chol.hpp:14-44)
- *   mpc_update_diag    <- update_diag            src/kernels_macros.cuh:177-190
+ *   mpc_apply_precond  <- the cusolverDnDpotrs calls of mixed_cholesky.cu:147,166,204
+ *   (update_diag, kernels_macros.cuh:177-190, is fused into mpc_factor: its
+ *    tracker comes back through d_updated_diag)
  *   mpc_rbf_* / mpc_gp_* : no reference counterpart (the RBF/GP workloads and
  *                         the backward pass are specified by the workload
  *                         definitions only; src/RBF.cu holds no RBF code).
@@ -116,7 +118,7 @@ int mpc_median_pairwise_distance(mpc_handle* h, const float* dX, int64_t n, int 
 
 /* ---- factorisation (switching_chol, switching_chol.cuh:19-227) --------- */
 typedef struct {
-    int    nb;              /* block size: 64, 128, 256 (default) or 512 */
+    int    nb;              /* block size: 64, 128 or 256 (default) */
     int    update_prec;     /* MPC_PREC_* for every tile, or MPC_PREC_SWITCH */
     int    ladder[4];       /* switching ladder, lowest precision first */
     int    n_ladder;
@@ -193,6 +195,11 @@ int mpc_quantize_panel(mpc_handle* h, const float* dP, int64_t m, int64_t k, int
 int mpc_syrk(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_t k,
              const uint8_t* d_payload, int64_t row_stride, const uint8_t* d_scales,
              int tile_rows, const uint8_t* d_prec_of_tile);
+/* z = (L L^T)^{-1} r for nrhs columns (the preconditioner application of the
+   PCG loop, the Dpotrs calls of mixed_cholesky.cu:147,166,204), FP64
+   vectors, FP32 lower factor L from mpc_factor.  R is not modified. */
+int mpc_apply_precond(mpc_handle* h, const float* dL, int64_t ldl, int64_t n, int nrhs,
+                      const double* dR, int64_t ldr, double* dZ, int64_t ldz);
 /* POTRF of one diagonal block (n <= 256) in place; host info result */
 int mpc_potrf_block(mpc_handle* h, float* dA, int n, int64_t lda, int* info);
 /* element-wise FP32 -> 8-bit conversion of a device array (prec E4M3 or E5M2),
