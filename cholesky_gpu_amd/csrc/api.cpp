@@ -59,11 +59,12 @@ __global__ void init_small_kernel(mpc_small* sm) {
         sm->nan_flag = 0;
         sm->mu_bits = 0x7f800000u;
     }
-    for (int t = threadIdx.x; t < 1024; t += blockDim.x) {
-        sm->tile_amax[t] = 0u;
-        sm->tile_maxrs[t] = 0u;
-        sm->tile_prec[t] = 0;
-    }
+    for (int t = threadIdx.x; t < 1024; t += blockDim.x)
+        for (int b = 0; b < 2; ++b) {
+            sm->tile_amax[b][t] = 0u;
+            sm->tile_maxrs[b][t] = 0u;
+            sm->tile_prec[b][t] = 0;
+        }
 }
 __global__ void init_factor_kernel(const float* A, int64_t lda, int64_t n, float* tracker,
                                    uint8_t* map, int64_t nt) {
@@ -108,8 +109,16 @@ extern "C" int mpc_handle_create(mpc_handle** out, int device) {
         return MPC_ERR_HIP;
     }
     h->own_stream = true;
-    int rc = mpc_ws_reserve(h->small, sizeof(mpc_small));
-    if (rc) { hipStreamDestroy(h->stream); delete h; return rc; }
+    e = hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_col, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming);
+    int rc = 0;
+    if (e != hipSuccess) {
+        mpc_set_error("stream/event creation failed: %s", hipGetErrorString(e));
+        rc = MPC_ERR_HIP;
+    }
+    if (!rc) rc = mpc_ws_reserve(h->small, sizeof(mpc_small));
+    if (rc) { mpc_handle_destroy(h); return rc; }
     *out = h;
     return 0;
 }
@@ -123,10 +132,14 @@ static void ws_free(mpc_workspace& w) {
 extern "C" int mpc_handle_destroy(mpc_handle* h) {
     if (!h) return 0;
     hipSetDevice(h->device);
-    hipStreamSynchronize(h->stream);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->pstream) hipStreamSynchronize(h->pstream);
     for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
                              &h->pcg, &h->scratch, &h->winv})
         ws_free(*w);
+    if (h->ev_col) hipEventDestroy(h->ev_col);
+    if (h->ev_panel) hipEventDestroy(h->ev_panel);
+    if (h->pstream) hipStreamDestroy(h->pstream);
     if (h->own_stream) hipStreamDestroy(h->stream);
     delete h;
     return 0;
@@ -435,8 +448,8 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     const int tr = tile_rows_for(nb);
     hipStream_t st = h->stream;
     int rc;
-    if ((rc = mpc_ws_reserve(h->panel, (size_t)(n + 128) * nb * 4))) return rc;
-    if ((rc = mpc_ws_reserve(h->scales, (size_t)(n + 128) * (nb / 32)))) return rc;
+    if ((rc = mpc_ws_reserve(h->panel, 2 * (size_t)(n + 128) * nb * 4))) return rc;
+    if ((rc = mpc_ws_reserve(h->scales, 2 * (size_t)(n + 128) * (nb / 32)))) return rc;
     if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nt * nb / 32 + 8) * 1024 * sizeof(float)))) return rc;
     if ((rc = mpc_ws_reserve(h->rowstat, (size_t)(n + 128) * sizeof(float)))) return rc;
     mpc_small* sm = (mpc_small*)h->small.ptr;
@@ -454,47 +467,93 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                        dA, lda, n, tracker, map, nt);
     MPC_LAUNCH_CHECK();
 
-    hipEvent_t ev[6] = {};
+    const int64_t pstride = (int64_t)(n + 128) * nb * 4;     // payload bytes per buffer
+    const int64_t sstride = (int64_t)(n + 128) * (nb / 32);  // scale bytes per buffer
+    // POTRF, TRSM and quantisation of panel j on stream `ps`, into the step-parity buffers
+    auto panel = [&](int64_t j, hipStream_t ps) -> int {
+        const int64_t c0 = j * nb, bs = std::min<int64_t>(nb, n - c0), e = c0 + bs, s = n - e;
+        const int ntl = (int)((s + tr - 1) / tr), b = (int)(j & 1);
+        float* Ajj = dA + c0 + c0 * lda;
+        int r;
+        if ((r = mpc_launch_potrf(ps, Ajj, lda, (int)bs, (int)c0, dinv, sm, ntl,
+                                  sm->tile_amax[b], sm->tile_maxrs[b])))
+            return r;
+        if (s == 0) return 0;
+        float* A21 = dA + e + c0 * lda;
+        if ((r = mpc_launch_trsm_stats(ps, A21, lda, s, nb, dinv + (c0 / 32) * 1024, Ajj,
+                                       rowss + e, tracker + e, tr, sm, sm->tile_amax[b],
+                                       sm->tile_maxrs[b])))
+            return r;
+        return mpc_launch_quantize_ex(ps, A21, s, nb, lda, tr, nullptr, sm->tile_amax[b],
+                                      sm->tile_maxrs[b], sm, o->update_prec, ladder, o->n_ladder,
+                                      eps, sm->tile_prec[b], map ? map + (j + 1) + j * nt : nullptr,
+                                      payload + b * pstride, (int64_t)nb * 4, scales + b * sstride,
+                                      &sm->nan_flag);
+    };
+    // trailing update of panel j restricted to SYRK tile columns [t0, t1) of A22
+    auto update = [&](int64_t j, int64_t t0, int64_t t1) -> int {
+        const int64_t e = std::min<int64_t>((j + 1) * nb, n), s = n - e;
+        const int b = (int)(j & 1);
+        return mpc_launch_syrk_classes(st, dA + e + e * lda, s, lda, nb, payload + b * pstride,
+                                       (int64_t)nb * 4, scales + b * sstride, tr,
+                                       sm->tile_prec[b], sm, cmask, fmask, t0, t1);
+    };
+
     const bool prof = o->profile != 0;
     double tp = 0, tt = 0, tq = 0, ts = 0;
-    if (prof)
-        for (auto& e : ev) MPC_HIP(hipEventCreate(&e));
     hipEvent_t e0, e1;
     MPC_HIP(hipEventCreate(&e0));
     MPC_HIP(hipEventCreate(&e1));
     MPC_HIP(hipEventRecord(e0, st));
-    for (int64_t j = 0; j < nt; ++j) {
-        const int64_t c0 = j * nb, bs = std::min<int64_t>(nb, n - c0), e = c0 + bs, s = n - e;
-        const int ntl = (int)((s + tr - 1) / tr);
-        float* Ajj = dA + c0 + c0 * lda;
-        if (prof) MPC_HIP(hipEventRecord(ev[0], st));
-        if ((rc = mpc_launch_potrf(st, Ajj, lda, (int)bs, (int)c0, dinv, sm, ntl))) return rc;
-        if (prof) MPC_HIP(hipEventRecord(ev[1], st));
-        if (s == 0) {
-            if (prof) {
-                MPC_HIP(hipEventSynchronize(ev[1]));
-                float ms;
-                hipEventElapsedTime(&ms, ev[0], ev[1]);
-                tp += ms;
-            }
-            break;
+    if (!prof) {
+        // one-deep look-ahead: once the first block column of A22 is updated,
+        // panel j+1 is factored on pstream while the rest of A22 is updated on
+        // st.  Panel j+1 touches only block column j+1 and the other parity's
+        // buffers; the rest of the update touches columns beyond it.
+        if ((rc = panel(0, st))) return rc;
+        for (int64_t j = 0; j + 1 < nt; ++j) {
+            const int64_t bs1 = std::min<int64_t>(nb, n - (j + 1) * nb);
+            const int64_t split = (bs1 + MPC_SYRK_TILE - 1) / MPC_SYRK_TILE;
+            if ((rc = update(j, 0, split))) return rc;
+            MPC_HIP(hipEventRecord(h->ev_col, st));
+            MPC_HIP(hipStreamWaitEvent(h->pstream, h->ev_col, 0));
+            if ((rc = panel(j + 1, h->pstream))) return rc;
+            MPC_HIP(hipEventRecord(h->ev_panel, h->pstream));
+            if ((rc = update(j, split, -1))) return rc;
+            MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
         }
-        float* A21 = dA + e + c0 * lda;
-        if ((rc = mpc_launch_trsm_stats(st, A21, lda, s, nb, dinv + (c0 / 32) * 1024, Ajj,
-                                        rowss + e, tracker + e, tr, sm)))
-            return rc;
-        if (prof) MPC_HIP(hipEventRecord(ev[2], st));
-        if ((rc = mpc_launch_quantize_ex(st, A21, s, nb, lda, tr, nullptr, sm->tile_amax, sm,
-                                         o->update_prec, ladder, o->n_ladder, eps, sm->tile_prec,
-                                         map ? map + (j + 1) + j * nt : nullptr, payload,
-                                         (int64_t)nb * 4, scales, &sm->nan_flag)))
-            return rc;
-        if (prof) MPC_HIP(hipEventRecord(ev[3], st));
-        if ((rc = mpc_launch_syrk_classes(st, dA + e + e * lda, s, lda, nb, payload,
-                                          (int64_t)nb * 4, scales, tr, sm->tile_prec, sm,
-                                          cmask, fmask)))
-            return rc;
-        if (prof) {
+    } else {
+        // sequential with per-phase events
+        hipEvent_t ev[5] = {};
+        for (auto& e : ev) MPC_HIP(hipEventCreate(&e));
+        for (int64_t j = 0; j < nt; ++j) {
+            const int64_t c0 = j * nb, bs = std::min<int64_t>(nb, n - c0), e = c0 + bs, s = n - e;
+            const int ntl = (int)((s + tr - 1) / tr), b = (int)(j & 1);
+            float* Ajj = dA + c0 + c0 * lda;
+            float* A21 = dA + e + c0 * lda;
+            MPC_HIP(hipEventRecord(ev[0], st));
+            if ((rc = mpc_launch_potrf(st, Ajj, lda, (int)bs, (int)c0, dinv, sm, ntl,
+                                       sm->tile_amax[b], sm->tile_maxrs[b])))
+                return rc;
+            MPC_HIP(hipEventRecord(ev[1], st));
+            if (s > 0) {
+                if ((rc = mpc_launch_trsm_stats(st, A21, lda, s, nb, dinv + (c0 / 32) * 1024, Ajj,
+                                                rowss + e, tracker + e, tr, sm, sm->tile_amax[b],
+                                                sm->tile_maxrs[b])))
+                    return rc;
+                MPC_HIP(hipEventRecord(ev[2], st));
+                if ((rc = mpc_launch_quantize_ex(
+                         st, A21, s, nb, lda, tr, nullptr, sm->tile_amax[b], sm->tile_maxrs[b], sm,
+                         o->update_prec, ladder, o->n_ladder, eps, sm->tile_prec[b],
+                         map ? map + (j + 1) + j * nt : nullptr, payload + b * pstride,
+                         (int64_t)nb * 4, scales + b * sstride, &sm->nan_flag)))
+                    return rc;
+                MPC_HIP(hipEventRecord(ev[3], st));
+                if ((rc = update(j, 0, -1))) return rc;
+            } else {
+                MPC_HIP(hipEventRecord(ev[2], st));
+                MPC_HIP(hipEventRecord(ev[3], st));
+            }
             MPC_HIP(hipEventRecord(ev[4], st));
             MPC_HIP(hipEventSynchronize(ev[4]));
             float ms;
@@ -503,6 +562,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
             hipEventElapsedTime(&ms, ev[2], ev[3]); tq += ms;
             hipEventElapsedTime(&ms, ev[3], ev[4]); ts += ms;
         }
+        for (auto& e : ev) hipEventDestroy(e);
     }
     MPC_HIP(hipEventRecord(e1, st));
     mpc_small hs;
@@ -512,8 +572,6 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     hipEventElapsedTime(&ms_total, e0, e1);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    if (prof)
-        for (auto& e : ev) hipEventDestroy(e);
     fi->ms_total += ms_total;
     fi->ms_potrf += tp;
     fi->ms_trsm += tt;
@@ -658,8 +716,10 @@ int apply_prec(mpc_handle* h, const float* L, int64_t ldl, int64_t n, int nb, in
     hipStream_t st = h->stream;
     int rc = mpc_launch_copy2d(st, n, nrhs, R, n, b.W, n);
     if (rc) return rc;
-    return mpc_apply_block_prec(st, L, ldl, n, h->winv_bo, (const float*)h->winv.ptr, b.W, b.Y, Z,
-                                n, nrhs);
+    const int64_t nI = (n + h->winv_bo - 1) / h->winv_bo;
+    const float* Wv = (const float*)h->winv.ptr;
+    return mpc_apply_block_prec(st, L, ldl, n, h->winv_bo, Wv, Wv + (size_t)nI * h->winv_bo * h->winv_bo,
+                                b.W, b.Y, Z, n, nrhs);
 }
 }  // namespace
 
@@ -679,9 +739,10 @@ static int prepare_prec(mpc_handle* h, const float* dL, int64_t ldl, int64_t n) 
     const int Bo = mpc_block_bo(n);
     if (h->winv_owner != dL || h->winv_n != n || h->winv_ld != ldl || h->winv_bo != Bo) {
         const int64_t nI = (n + Bo - 1) / Bo;
-        if ((rc = mpc_ws_reserve(h->winv, sizeof(float) * (size_t)nI * Bo * Bo))) return rc;
+        const size_t blk = (size_t)nI * Bo * Bo;
+        if ((rc = mpc_ws_reserve(h->winv, sizeof(float) * 2 * blk))) return rc;
         if ((rc = mpc_build_block_inverses(st, dL, ldl, n, Bo, (const float*)h->dinv.ptr,
-                                           (float*)h->winv.ptr)))
+                                           (float*)h->winv.ptr, (float*)h->winv.ptr + blk)))
             return rc;
         h->winv_owner = dL;
         h->winv_n = n;
@@ -705,9 +766,9 @@ static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL
     hipStream_t st = h->stream;
     int rc;
     if ((rc = prepare_prec(h, dL, ldl, n))) return rc;
-    const int ns = mpc_gemv_nsplit(n);
     const size_t vec = (size_t)n * nrhs;
-    const size_t part = std::max<size_t>((size_t)ns * vec, (size_t)nrhs * ((n + 2047) / 2048) + 64);
+    const size_t part = std::max<size_t>((size_t)mpc_symv_partial_doubles(n, nrhs),
+                                         (size_t)nrhs * ((n + 2047) / 2048) + 64);
     const size_t nbytes = sizeof(double) * (6 * vec + part + 5 * (size_t)nrhs + 64) + sizeof(int) * nrhs + 1024;
     if ((rc = mpc_ws_reserve(h->pcg, nbytes))) return rc;
     pcg_bufs b;
@@ -735,7 +796,7 @@ static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL
         if ((rc = mpc_launch_axpby(st, n, nrhs, 0.0, dX, ldx, 0.0, dX, ldx))) return rc;
     }
     // R = B - A X
-    if ((rc = mpc_launch_gemv(st, dA, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
+    if ((rc = mpc_launch_symv(st, dA, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
     if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.R, n))) return rc;
     if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.R, n))) return rc;
     if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return rc;
@@ -767,7 +828,7 @@ static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL
             MPC_HIP(hipMemcpyAsync(b.active, act.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
             act_dirty = false;
         }
-        if ((rc = mpc_launch_gemv(st, dA, lda, n, nrhs, b.P, n, b.Q, n, b.part))) return rc;
+        if ((rc = mpc_launch_symv(st, dA, lda, n, nrhs, b.P, n, b.Q, n, b.part))) return rc;
         if ((rc = mpc_launch_dot(st, n, nrhs, b.P, n, b.Q, n, b.pq, b.part))) return rc;
         if ((rc = mpc_launch_pcg_update_xr(st, n, nrhs, dX, ldx, b.R, n, b.P, n, b.Q, n, b.rz, b.pq,
                                            b.active)))
@@ -798,7 +859,7 @@ static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL
         for (int j = 0; j < nrhs; ++j) hrz[j] = act[j] ? hrzn[j] : hrz[j];
         if (need_true && o->stop_rule == 0) {
             // confirm on the true residual B - A X; replace R where it lags
-            if ((rc = mpc_launch_gemv(st, dA, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
+            if ((rc = mpc_launch_symv(st, dA, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
             if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.W, n))) return rc;
             if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.W, n))) return rc;
             if ((rc = mpc_launch_dot(st, n, nrhs, b.W, n, b.W, n, b.rr, b.part))) return rc;
@@ -916,7 +977,7 @@ extern "C" int mpc_quantize_panel(mpc_handle* h, const float* dP, int64_t m, int
     MPC_HIP(hipMemsetAsync(h->scratch.ptr, 0, sizeof(uint32_t) * nt + sizeof(int) * 4, h->stream));
     if ((rc = mpc_launch_tile_amax(h->stream, dP, m, (int)k, ld, tile_rows, amax))) return rc;
     if ((rc = mpc_launch_quantize_ex(h->stream, dP, m, (int)k, ld, tile_rows, d_prec_of_tile, amax,
-                                     nullptr, 0, nullptr, 0, 0.0, nullptr, nullptr, d_payload,
+                                     nullptr, nullptr, 0, nullptr, 0, 0.0, nullptr, nullptr, d_payload,
                                      row_stride, d_scales, nanf)))
         return rc;
     int hn = 0;
@@ -953,8 +1014,10 @@ extern "C" int mpc_apply_precond(mpc_handle* h, const float* dL, int64_t ldl, in
     double* Z = Y + vec;
     hipStream_t st = h->stream;
     if ((rc = mpc_launch_copy2d(st, n, nrhs, dR, ldr, W, n))) return rc;
-    if ((rc = mpc_apply_block_prec(st, dL, ldl, n, h->winv_bo, (const float*)h->winv.ptr, W, Y, Z,
-                                   n, nrhs)))
+    const int64_t nI = (n + h->winv_bo - 1) / h->winv_bo;
+    const float* Wv = (const float*)h->winv.ptr;
+    if ((rc = mpc_apply_block_prec(st, dL, ldl, n, h->winv_bo, Wv,
+                                   Wv + (size_t)nI * h->winv_bo * h->winv_bo, W, Y, Z, n, nrhs)))
         return rc;
     if ((rc = mpc_launch_copy2d(st, n, nrhs, Z, n, dZ, ldz))) return rc;
     MPC_HIP(hipStreamSynchronize(st));
@@ -969,7 +1032,9 @@ extern "C" int mpc_potrf_block(mpc_handle* h, float* dA, int n, int64_t lda, int
     if (rc) return rc;
     mpc_small* sm = (mpc_small*)h->small.ptr;
     hipLaunchKernelGGL(init_small_kernel, dim3(1), dim3(256), 0, h->stream, sm);
-    if ((rc = mpc_launch_potrf(h->stream, dA, lda, n, 0, (float*)h->dinv.ptr, sm, 0))) return rc;
+    if ((rc = mpc_launch_potrf(h->stream, dA, lda, n, 0, (float*)h->dinv.ptr, sm, 0,
+                               sm->tile_amax[0], sm->tile_maxrs[0])))
+        return rc;
     int hi = 0;
     MPC_HIP(hipMemcpyAsync(&hi, &sm->info, sizeof(int), hipMemcpyDeviceToHost, h->stream));
     MPC_HIP(hipStreamSynchronize(h->stream));

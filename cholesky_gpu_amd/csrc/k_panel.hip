@@ -5,7 +5,7 @@
 //    fp8_chol.cuh:73-85; switching_chol.cuh:98-110 calls a LEFT form that is
 //    out of bounds).  One wave owns a 32-row slab and substitutes over the
 //    32-column blocks of the panel with the inverses of L_jj's 32x32 diagonal
-//    sub-blocks (MFMA 32x32x2 f32).  The epilogue computes each row's sum of
+//    sub-blocks (MFMA 32x32x2 f32) plus one refinement step per block.  The epilogue computes each row's sum of
 //    squares, subtracts it from the Schur-diagonal tracker (update_diag,
 //    kernels_macros.cuh:177-190) and folds the tile amax / max row sum of
 //    squares into per-tile slots with order-independent atomics.
@@ -30,7 +30,9 @@ __global__ __launch_bounds__(256) void trsm_stats_kernel(float* __restrict__ A21
                                                          const float* __restrict__ Ljj,
                                                          float* __restrict__ rowss,
                                                          float* __restrict__ tracker, int tile_rows,
-                                                         mpc_small* __restrict__ sm) {
+                                                         mpc_small* __restrict__ sm,
+                                                         uint32_t* __restrict__ t_amax,
+                                                         uint32_t* __restrict__ t_maxrs) {
     if (sm->info != 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * 32;
@@ -61,6 +63,22 @@ __global__ __launch_bounds__(256) void trsm_stats_kernel(float* __restrict__ A21
         for (int t = 0; t < 16; ++t) {
             float a = Db[n + sigma(t, h) * 32];
             xb = __builtin_amdgcn_mfma_f32_32x32x2f32(a, acc[t], xb, 0, 0, 0);
+        }
+        // one refinement step, r = acc - xb L_bb^T, xb += r Dinv_b^T: the
+        // product with the explicit inverse alone carries an error that grows
+        // with cond(L_bb); after one correction the result matches
+        // substitution (the strict upper part of L_jj holds A and is masked)
+        v16f rr = acc;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            const int q = sigma(t, h);
+            float a = q <= n ? -Ljj[(32 * b + n) + (int64_t)(32 * b + q) * lda] : 0.f;
+            rr = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xb[t], rr, 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            float a = Db[n + sigma(t, h) * 32];
+            xb = __builtin_amdgcn_mfma_f32_32x32x2f32(a, rr[t], xb, 0, 0, 0);
         }
         X[b] = xb;
         if (valid) {
@@ -93,8 +111,8 @@ __global__ __launch_bounds__(256) void trsm_stats_kernel(float* __restrict__ A21
     }
     if (lane == 0) {
         int t = (int)(r0 / tile_rows);
-        atomicMax(&sm->tile_maxrs[t], __float_as_uint(wss));
-        atomicMax(&sm->tile_amax[t], __float_as_uint(wam));
+        atomicMax(&t_maxrs[t], __float_as_uint(wss));
+        atomicMax(&t_amax[t], __float_as_uint(wam));
     }
 }
 
@@ -212,7 +230,7 @@ __global__ __launch_bounds__(256) void quantize_kernel(
 
 int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, int nb,
                           const float* dinv_j, const float* Ljj, float* rowss, float* tracker,
-                          int tile_rows, mpc_small* sm) {
+                          int tile_rows, mpc_small* sm, uint32_t* tile_amax, uint32_t* tile_maxrs) {
     if (s <= 0) return 0;
     if (tile_rows % 32) {
         mpc_set_error("trsm: tile_rows %d not a multiple of 32", tile_rows);
@@ -222,15 +240,15 @@ int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, in
     switch (nb) {
     case 64:
         hipLaunchKernelGGL(trsm_stats_kernel<2>, grid, dim3(256), 0, st, A21, lda, s, dinv_j, Ljj,
-                           rowss, tracker, tile_rows, sm);
+                           rowss, tracker, tile_rows, sm, tile_amax, tile_maxrs);
         break;
     case 128:
         hipLaunchKernelGGL(trsm_stats_kernel<4>, grid, dim3(256), 0, st, A21, lda, s, dinv_j, Ljj,
-                           rowss, tracker, tile_rows, sm);
+                           rowss, tracker, tile_rows, sm, tile_amax, tile_maxrs);
         break;
     case 256:
         hipLaunchKernelGGL(trsm_stats_kernel<8>, grid, dim3(256), 0, st, A21, lda, s, dinv_j, Ljj,
-                           rowss, tracker, tile_rows, sm);
+                           rowss, tracker, tile_rows, sm, tile_amax, tile_maxrs);
         break;
     default:
         mpc_set_error("trsm: unsupported block size %d", nb);
@@ -252,9 +270,10 @@ int mpc_launch_tile_amax(hipStream_t st, const float* P, int64_t m, int k, int64
 // factor path: decision from the tile statistics in sm; exported path: prec_in.
 int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
                            int tile_rows, const uint8_t* prec_in, const uint32_t* tile_amax,
-                           mpc_small* sm, int update_prec, const int* ladder, int n_ladder,
-                           double eps_prime, uint8_t* prec_out, uint8_t* map_col,
-                           uint8_t* payload, int64_t row_stride, uint8_t* scales, int* nan_flag) {
+                           const uint32_t* tile_maxrs, mpc_small* sm, int update_prec,
+                           const int* ladder, int n_ladder, double eps_prime, uint8_t* prec_out,
+                           uint8_t* map_col, uint8_t* payload, int64_t row_stride, uint8_t* scales,
+                           int* nan_flag) {
     if (m <= 0) return 0;
     if (k % 32 || tile_rows % 32) {
         mpc_set_error("quantize: k (%d) and tile_rows (%d) must be multiples of 32", k, tile_rows);
@@ -266,7 +285,7 @@ int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int
     for (int i = 0; i < 4; ++i) ra.ladder[i] = i < n_ladder ? ladder[i] : 0;
     ra.eps_prime = eps_prime;
     hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)((m + 31) / 32)), dim3(256), 0, st, P, m, k,
-                       ld, tile_rows, prec_in, tile_amax, sm ? sm->tile_maxrs : nullptr, sm, ra,
+                       ld, tile_rows, prec_in, tile_amax, tile_maxrs, sm, ra,
                        prec_out, map_col, payload, row_stride, scales, nan_flag);
     MPC_LAUNCH_CHECK();
     return 0;

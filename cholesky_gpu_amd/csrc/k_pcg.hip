@@ -70,6 +70,75 @@ __global__ __launch_bounds__(RB) void gemv_reduce_kernel(int64_t n, int nrhs, in
     Y[(int64_t)j * ldy + r] = s;
 }
 
+// ---------------------------------------------------------------- SYMV
+// y = A x for symmetric A reading only the upper triangle (i <= j) of the
+// column-major FP32 matrix: half the HBM traffic of a full GEMV.  One
+// workgroup per 128 x 128 tile (I <= J), staged in LDS once; its row sums
+// (A_IJ x_J) and column sums (A_IJ^T x_I) go to fixed partial slots
+//   P[K][L][r]:  L >= K row-sum partial of tile (K, L); L < K column-sum
+//               partial of tile (L, K),
+// and symv_reduce sums P[K][0..nt) in order (deterministic).
+constexpr int ST = 128;
+
+__global__ __launch_bounds__(256) void symv_tile_kernel(const float* __restrict__ A, int64_t lda,
+                                                       int64_t n, int nt, int nrhs,
+                                                       const double* __restrict__ X, int64_t ldx,
+                                                       double* __restrict__ P) {
+    __shared__ float As[ST * (ST + 1)];
+    __shared__ double xi[ST], xj[ST];
+    // tile index -> (I, J), I <= J, row-major over the upper triangle
+    const int64_t t = blockIdx.x;
+    int64_t I = (int64_t)((2.0 * nt + 1.0 - sqrt((2.0 * nt + 1.0) * (2.0 * nt + 1.0) - 8.0 * t)) * 0.5);
+    auto start = [&](int64_t i) { return i * nt - i * (i - 1) / 2; };
+    while (I > 0 && start(I) > t) --I;
+    while (start(I + 1) <= t) ++I;
+    const int64_t J = I + (t - start(I));
+    const int tid = threadIdx.x;
+    const int64_t r0 = I * ST, c0 = J * ST;
+    const bool diag = I == J;
+    for (int idx = tid; idx < ST * ST; idx += 256) {
+        int r = idx & (ST - 1), c = idx >> 7;
+        int64_t gr = r0 + r, gc = c0 + c;
+        float v = 0.f;
+        if (gr < n && gc < n && (!diag || r <= c)) v = A[gr + gc * lda];
+        As[c * (ST + 1) + r] = v;
+    }
+    for (int j = 0; j < nrhs; ++j) {
+        if (tid < ST) {
+            xi[tid] = r0 + tid < n ? X[j * ldx + r0 + tid] : 0.0;
+            xj[tid] = c0 + tid < n ? X[j * ldx + c0 + tid] : 0.0;
+        }
+        __syncthreads();
+        // threads 0..127: row sums of row tid; 128..255: column sums of column tid-128
+        double s = 0.0;
+        if (tid < ST) {
+            const int r = tid;
+            for (int c = 0; c < ST; ++c) s = fma((double)As[c * (ST + 1) + r], xj[c], s);
+            if (diag) {   // mirrored strict lower part: sum_{c < r} A[c][r] x[c]
+                for (int c = 0; c < r; ++c) s = fma((double)As[r * (ST + 1) + c], xi[c], s);
+            }
+            P[((I * nt + J) * nrhs + j) * ST + r] = s;
+        } else if (!diag) {
+            const int c = tid - ST;
+            for (int r = 0; r < ST; ++r) s = fma((double)As[c * (ST + 1) + r], xi[r], s);
+            P[((J * nt + I) * nrhs + j) * ST + c] = s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void symv_reduce_kernel(int64_t n, int nt, int nrhs,
+                                                         const double* __restrict__ P,
+                                                         double* __restrict__ Y, int64_t ldy) {
+    const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (row >= n) return;
+    const int64_t K = row / ST, r = row % ST;
+    double s = 0.0;
+    for (int64_t L = 0; L < nt; ++L) s += P[((K * nt + L) * nrhs + j) * ST + r];
+    Y[j * ldy + row] = s;
+}
+
 // ---------------------------------------------------------------- TRSV steps
 // Solve the diagonal block in LDS: y = L_jj^{-1} w (forward) or
 // L_jj^{-T} w (backward) via the 32x32 inverses, for one right-hand side.
@@ -290,6 +359,24 @@ int mpc_launch_gemv(hipStream_t st, const float* A, int64_t lda, int64_t n, int 
     MPC_LAUNCH_CHECK();
     hipLaunchKernelGGL(gemv_reduce_kernel, dim3(blocks_for(n), (unsigned)nrhs), dim3(RB), 0, st, n,
                        nrhs, ns, part, Y, ldy);
+    MPC_LAUNCH_CHECK();
+    return 0;
+}
+
+int64_t mpc_symv_partial_doubles(int64_t n, int nrhs) {
+    const int64_t nt = (n + ST - 1) / ST;
+    return nt * nt * nrhs * ST;
+}
+
+int mpc_launch_symv(hipStream_t st, const float* A, int64_t lda, int64_t n, int nrhs,
+                    const double* X, int64_t ldx, double* Y, int64_t ldy, double* part) {
+    const int64_t nt = (n + ST - 1) / ST;
+    const int64_t ntile = nt * (nt + 1) / 2;
+    hipLaunchKernelGGL(symv_tile_kernel, dim3((unsigned)ntile), dim3(256), 0, st, A, lda, n, (int)nt,
+                       nrhs, X, ldx, part);
+    MPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(symv_reduce_kernel, dim3(blocks_for(n), (unsigned)nrhs), dim3(256), 0, st, n,
+                       (int)nt, nrhs, part, Y, ldy);
     MPC_LAUNCH_CHECK();
     return 0;
 }

@@ -7,7 +7,8 @@
 // bs = 256); the algorithm is right-looking over 32-column sub-panels:
 //   1. wave 0 factors the diagonal sub-tile in registers (row per lane) and
 //      inverts it (column per lane);
-//   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32);
+//   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32)
+//      with one refinement step against the factored sub-tile;
 //   3. the trailing sub-tiles receive the rank-32 update (MFMA 32x32x2 f32).
 // Outputs: L in place (lower; the strict upper triangle of A is untouched),
 // dinv (column-major 32x32 inverse per sub-block), the running minimum pivot
@@ -24,6 +25,9 @@ constexpr int NSUB = MAXSB * (MAXSB + 1) / 2;
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ int sidx(int bi, int bj) { return bi * (bi + 1) / 2 + bj; }
+
+// row of accumulator register i in lane half h (32x32 MFMA output layout)
+__device__ __forceinline__ int sig(int i, int h) { return 8 * (i >> 2) + 4 * h + (i & 3); }
 
 __device__ __forceinline__ float rdlane(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
@@ -47,7 +51,9 @@ __device__ __forceinline__ v16f mfma_32x32x32_lds(v16f acc, const float* a, int 
 
 __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64_t lda, int bs,
                                                     int col0, float* __restrict__ dinv,
-                                                    mpc_small* __restrict__ sm, int ntiles_next) {
+                                                    mpc_small* __restrict__ sm, int ntiles_next,
+                                                    uint32_t* __restrict__ st_amax,
+                                                    uint32_t* __restrict__ st_maxrs) {
     __shared__ float T[NSUB * SUBSZ];
     __shared__ float Dinv[SUBSZ];
     __shared__ int fail_col;
@@ -56,27 +62,36 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (sm->info != 0) return;                       // an earlier step failed
     for (int t = tid; t < ntiles_next; t += 256) {
-        sm->tile_amax[t] = 0u;
-        sm->tile_maxrs[t] = 0u;
+        st_amax[t] = 0u;
+        st_maxrs[t] = 0u;
     }
     if (tid == 0) fail_col = -1;
     const int nsb = (bs + 31) >> 5;
 
-    // ---- load lower sub-tiles (padding: identity beyond bs)
-    for (int bi = 0; bi < nsb; ++bi)
-        for (int bj = 0; bj <= bi; ++bj) {
-            float* S = T + sidx(bi, bj) * SUBSZ;
-            for (int idx = tid; idx < 1024; idx += 256) {
-                int r = idx & 31, c = idx >> 5;
-                int gr = bi * 32 + r, gc = bj * 32 + c;
-                float v;
-                if (gr < bs && gc < bs)
-                    v = (gr >= gc) ? A[gr + (int64_t)gc * lda] : 0.f;
-                else
-                    v = (gr == gc) ? 1.f : 0.f;
-                S[r + c * SUBLD] = v;
+    // ---- load lower sub-tiles (padding: identity beyond bs).  Thread = row,
+    // columns in unrolled batches so LB loads per thread are in flight at once
+    // (one batch per HBM latency instead of one load per latency).
+    {
+        constexpr int LB = 32;
+        const int r = tid, rb = r >> 5;
+        const int ncol = nsb * 32;
+        for (int c0 = 0; c0 < ncol; c0 += LB) {
+            float v[LB];
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int c = c0 + u;
+                float x = (r == c) ? 1.f : 0.f;
+                if (r < bs && c < bs && r >= c && rb < nsb) x = A[r + (int64_t)c * lda];
+                v[u] = x;
+            }
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int c = c0 + u, cb = c >> 5;
+                if (rb < nsb && rb >= cb)
+                    T[sidx(rb, cb) * SUBSZ + (r & 31) + (c & 31) * SUBLD] = (r >= c) ? v[u] : 0.f;
             }
         }
+    }
     __syncthreads();
 
     for (int kb = 0; kb < nsb; ++kb) {
@@ -104,10 +119,12 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
                     if (r >= c) d[c] = fmaf(-d[k], lck, d[c]);
                 }
             }
+            // the whole sub-tile, zeros above the diagonal (d[c] = 0 for c > r):
+            // the trailing updates of earlier kb wrote the full 32x32 block,
+            // and the refinement below reads D as a lower-triangular operand
             if (lane < 32) {
 #pragma unroll
-                for (int c = 0; c < 32; ++c)
-                    if (c <= r) D[r + c * SUBLD] = d[c];
+                for (int c = 0; c < 32; ++c) D[r + c * SUBLD] = d[c];
             }
             if (bad >= 0 && lane == 0 && kb * 32 + bad < bs && fail_col < 0)
                 fail_col = kb * 32 + bad;
@@ -132,18 +149,32 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         }
         __syncthreads();
 
-        // ---- 2. sub-tiles below: S <- S * Dinv^T   (out(r,c) = sum_q S(r,q) Dinv(c,q))
+        // ---- 2. sub-tiles below: X = S * Dinv^T, refined once:
+        //   R = S - X D^T, X += R Dinv^T  (D the factored diagonal sub-tile,
+        //   zero above its diagonal).  Worked in the transposed domain: lane
+        //   (n, h) register i holds X(n, sig(i, h)), the k index of MFMA step t
+        //   is sig(t, h) on both operands, so no LDS round trip is needed.
         for (int ib = kb + 1 + wave; ib < nsb; ib += 4) {
             float* S = T + sidx(ib, kb) * SUBSZ;
-            v16f acc = {};
-            // A(m=r,k=q) = S[r + q*ld]; B(k=q,n=c) = Dinv(c,q) = Dinv[c + q*ld]
-            acc = mfma_32x32x32_lds(acc, S, SUBLD, Dinv, SUBLD, 1.f);
             const int n = lane & 31, h = lane >> 5;
+            v16f s0, x = {};
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                int m = 8 * (i >> 2) + 4 * h + (i & 3);
-                S[m + n * SUBLD] = acc[i];
-            }
+            for (int i = 0; i < 16; ++i) s0[i] = S[n + sig(i, h) * SUBLD];
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                x = __builtin_amdgcn_mfma_f32_32x32x2f32(Dinv[n + sig(t, h) * SUBLD], s0[t], x,
+                                                         0, 0, 0);
+            v16f r = s0;
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                r = __builtin_amdgcn_mfma_f32_32x32x2f32(-D[n + sig(t, h) * SUBLD], x[t], r, 0, 0,
+                                                         0);
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                x = __builtin_amdgcn_mfma_f32_32x32x2f32(Dinv[n + sig(t, h) * SUBLD], r[t], x, 0,
+                                                         0, 0);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) S[n + sig(i, h) * SUBLD] = x[i];
         }
         __syncthreads();
 
@@ -179,21 +210,19 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         __syncthreads();
     }
 
-    // ---- write back the lower triangle, min pivot^2, failure
+    // ---- write back the lower triangle (thread = row), min pivot^2, failure
     float pmin = INFINITY;
-    for (int bi = 0; bi < nsb; ++bi)
-        for (int bj = 0; bj <= bi; ++bj) {
-            const float* S = T + sidx(bi, bj) * SUBSZ;
-            for (int idx = tid; idx < 1024; idx += 256) {
-                int r = idx & 31, c = idx >> 5;
-                int gr = bi * 32 + r, gc = bj * 32 + c;
-                if (gr < bs && gc < bs && gr >= gc) {
-                    float v = S[r + c * SUBLD];
-                    A[gr + (int64_t)gc * lda] = v;
-                    if (gr == gc) pmin = fminf(pmin, v * v);
-                }
+    {
+        const int r = tid, rb = r >> 5;
+        if (r < bs) {
+            for (int c = 0; c <= r; ++c) {
+                const float v = T[sidx(rb, c >> 5) * SUBSZ + (r & 31) + (c & 31) * SUBLD];
+                A[r + (int64_t)c * lda] = v;
             }
+            const float d = T[sidx(rb, rb) * SUBSZ + (r & 31) * (1 + SUBLD)];
+            pmin = d * d;
         }
+    }
     red[tid] = pmin;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -212,7 +241,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
 }  // namespace
 
 int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0, float* dinv,
-                     mpc_small* sm, int ntiles_next) {
+                     mpc_small* sm, int ntiles_next, uint32_t* stats_amax, uint32_t* stats_maxrs) {
     if (bs <= 0 || bs > 256) {
         mpc_set_error("potrf: block size %d outside 1..256", bs);
         return MPC_ERR_ARG;
@@ -222,7 +251,7 @@ int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0, fl
         return MPC_ERR_ARG;
     }
     hipLaunchKernelGGL(potrf_kernel, dim3(1), dim3(256), 0, st, A, lda, bs, col0, dinv, sm,
-                       ntiles_next);
+                       ntiles_next, stats_amax, stats_maxrs);
     MPC_LAUNCH_CHECK();
     return 0;
 }

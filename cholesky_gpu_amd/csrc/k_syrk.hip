@@ -120,33 +120,12 @@ struct tile_ctx {
 // Element (a, b, i) of the wave's quadrant sits at wave base + 32-bit offset
 //   (32a + n) + (32b + 8(i/4) + 4h + i%4) * ldc,
 // which stays below 2^31 for ldc <= 2^24; only the base pointer is 64-bit.
-__device__ __forceinline__ void load_c(const tile_ctx& t, v16f acc[2][2]) {
+// The product Q Q^T is accumulated from zero and subtracted from C once, as a
+// BLAS GEMM with beta = 1 does: seeding the accumulator with -C instead would
+// round every one of the k FMA steps at the scale of |C| rather than of the
+// partial sum, about sqrt(k) times the error for a trailing update.
+__device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[2][2]) {
     const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
-    const float* base = t.C + t.rowbase + t.colbase * t.ldc;
-    const int ld = (int)t.ldc;
-    const int rlim = (int)(t.m - t.rowbase), clim = (int)(t.m - t.colbase);
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int r = 32 * a + n;
-#pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
-                float v = 0.f;
-                const bool ok = r < rlim && c < clim && (!t.diag || r >= c);
-                if (ok) v = base[r + c * ld];
-                acc[a][b][i] = -v;
-            }
-        }
-}
-
-__device__ __forceinline__ void store_c(const tile_ctx& t, v16f acc[2][2]) {
-    // opaque copy of the lane id: the store recomputes its offsets and masks
-    // instead of keeping load_c's 64 of them live across the MFMA loop
-    int lane = threadIdx.x & 63;
-    asm volatile("" : "+v"(lane));
-    const int h = lane >> 5, n = lane & 31;
     float* base = t.C + t.rowbase + t.colbase * t.ldc;
     const int ld = (int)t.ldc;
     const int rlim = (int)(t.m - t.rowbase), clim = (int)(t.m - t.colbase);
@@ -155,11 +134,18 @@ __device__ __forceinline__ void store_c(const tile_ctx& t, v16f acc[2][2]) {
 #pragma unroll
         for (int b = 0; b < 2; ++b) {
             const int r = 32 * a + n;
+            float v[16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
                 const bool ok = r < rlim && c < clim && (!t.diag || r >= c);
-                if (ok) base[r + c * ld] = -acc[a][b][i];
+                v[i] = ok ? base[r + c * ld] : 0.f;
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
+                const bool ok = r < rlim && c < clim && (!t.diag || r >= c);
+                if (ok) base[r + c * ld] = v[i] - acc[a][b][i];
             }
         }
 }
@@ -279,16 +265,22 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
                                                    int tile_rows,
                                                    const uint8_t* __restrict__ prec_of_tile,
                                                    const mpc_small* __restrict__ sm, int64_t npair,
-                                                   int64_t per_xcd) {
+                                                   int64_t per_xcd, int64_t mt, int64_t tcol0) {
     if (sm && sm->info != 0) return;
     // XCD-aware remap: hardware places block b on XCD b % 8
     const int64_t b = blockIdx.x;
     const int64_t pid = (b & 7) * per_xcd + (b >> 3);
     if (pid >= npair) return;
-    int64_t ti = (int64_t)((sqrt(8.0 * (double)pid + 1.0) - 1.0) * 0.5);
-    while (ti * (ti + 1) / 2 > pid) --ti;
-    while ((ti + 1) * (ti + 2) / 2 <= pid) ++ti;
-    const int64_t tj = pid - ti * (ti + 1) / 2;
+    // column-major over the lower triangle from tile column tcol0: column c
+    // holds tiles ti = c .. mt-1; start(c) = c*mt - c*(c-1)/2
+    auto start = [&](int64_t c) { return c * mt - c * (c - 1) / 2; };
+    const int64_t gid = pid + start(tcol0);
+    const double tw = 2.0 * (double)mt + 1.0;
+    int64_t tj = (int64_t)((tw - sqrt(tw * tw - 8.0 * (double)gid)) * 0.5);
+    if (tj < 0) tj = 0;
+    while (tj > 0 && start(tj) > gid) --tj;
+    while (start(tj + 1) <= gid) ++tj;
+    const int64_t ti = tj + (gid - start(tj));
 
     operand_src orow, ocol;
     orow.prec = prec_of_tile[(ti * TILE) / tile_rows];
@@ -317,7 +309,10 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
     orow.kb = ocol.kb = k >> 5;
 
     v16f acc[2][2];
-    load_c(t, acc);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = v16f{};
     if constexpr (CLS == MPC_PREC_E4M3) {
         mm_fp8<FA, FB>(ocol, orow, t, k, acc);
     } else if constexpr (CLS == MPC_PREC_BF16) {
@@ -327,7 +322,7 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
     } else {
         mm_f32(ocol, orow, t, k, acc);
     }
-    store_c(t, acc);
+    update_c(t, acc);
 }
 
 }  // namespace
@@ -339,7 +334,7 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
 int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                             const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                             int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
-                            unsigned class_mask, unsigned fp8_mask) {
+                            unsigned class_mask, unsigned fp8_mask, int64_t tcol0, int64_t tcol1) {
     if (m <= 0) return 0;
     if (k % 64 || tile_rows % TILE) {
         mpc_set_error("syrk: k (%d) must be a multiple of 64 and tile_rows (%d) of 128", k,
@@ -347,12 +342,15 @@ int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, in
         return MPC_ERR_ARG;
     }
     const int64_t mt = (m + TILE - 1) / TILE;
-    const int64_t npair = mt * (mt + 1) / 2;
+    if (tcol1 < 0 || tcol1 > mt) tcol1 = mt;
+    if (tcol0 >= tcol1) return 0;
+    auto start = [&](int64_t c) { return c * mt - c * (c - 1) / 2; };
+    const int64_t npair = start(tcol1) - start(tcol0);
     const int64_t per_xcd = (npair + 7) / 8;
     dim3 grid((unsigned)(per_xcd * 8));
 #define MPC_SYRK_LAUNCH(CLS, FA, FB)                                                             \
     hipLaunchKernelGGL((syrk_kernel<CLS, FA, FB>), grid, dim3(256), 0, st, C, m, ldc, k, payload, \
-                       row_stride, scales, tile_rows, prec_of_tile, sm, npair, per_xcd)
+                       row_stride, scales, tile_rows, prec_of_tile, sm, npair, per_xcd, mt, tcol0)
     if (class_mask & (1u << MPC_PREC_E4M3)) {
         const bool e4 = fp8_mask & 1u, e5 = fp8_mask & 2u;
         if (e4) MPC_SYRK_LAUNCH(MPC_PREC_E4M3, 0, 0);
@@ -373,5 +371,5 @@ int mpc_launch_syrk(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
     const unsigned all = (1u << MPC_PREC_E4M3) | (1u << MPC_PREC_BF16) | (1u << MPC_PREC_FP16) |
                          (1u << MPC_PREC_FP32);
     return mpc_launch_syrk_classes(st, C, m, ldc, k, payload, row_stride, scales, tile_rows,
-                                   prec_of_tile, sm, all, 3u);
+                                   prec_of_tile, sm, all, 3u, 0, -1);
 }

@@ -24,43 +24,50 @@ namespace {
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------- batched GEMM
-// For batch b and output tile (64 x 64 at m0, n0):
+// For batch (q, I) and an output tile (T x T at m0, n0):
 //   C[m][n] = alpha * sum_k A[m][k] B[k][n]    (column-major, ld per operand)
-// over k in [klo, khi) where the triangular structure allows:
-//   a_lower: A[m][k] = 0 for k > m  -> khi <= m0 + 64
-//   b_lower: B[k][n] = 0 for k < n  -> klo >= n0
+// a_lower: A[m][k] = 0 for k > m; b_lower: B[k][n] = 0 for k < n.  Both are
+// applied per element while staging (the strict upper parts of inverse
+// blocks hold temporaries of the previous doubling level) and per tile when
+// choosing the K range.  Batch b = blockIdx.z = I * nq + q.
 struct gemm_batch {
     const float* A; const float* B; float* C;
     int64_t lda, ldb, ldc;
-    int64_t sA, sB, sC;          // batch strides (elements)
+    int64_t sA, sB, sC;          // stride between pairs q (elements)
+    int64_t tA, tB, tC;          // stride between big blocks I
+    int nq;
     int m, n, k;
     float alpha;
     int a_lower, b_lower;
 };
 
+template <int T>
 __global__ __launch_bounds__(256) void gemm_f32_kernel(gemm_batch g) {
-    __shared__ float As[32][64 + 4];   // As[k][m]
-    __shared__ float Bs[32][64 + 4];   // Bs[k][n]
-    const int64_t bidx = blockIdx.z;
-    const float* A = g.A + bidx * g.sA;
-    const float* B = g.B + bidx * g.sB;
-    float* C = g.C + bidx * g.sC;
-    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;
+    constexpr int WT = T / 2;          // per-wave tile (2 x 2 waves)
+    constexpr int NB = WT / 32;        // 32x32 MFMA blocks per wave side
+    __shared__ float As[32][T + 4];    // As[k][m]
+    __shared__ float Bs[32][T + 4];    // Bs[k][n]
+    const int64_t q = blockIdx.z % g.nq, I = blockIdx.z / g.nq;
+    const float* A = g.A + q * g.sA + I * g.tA;
+    const float* B = g.B + q * g.sB + I * g.tB;
+    float* C = g.C + q * g.sC + I * g.tC;
+    const int m0 = blockIdx.x * T, n0 = blockIdx.y * T;
     if (m0 >= g.m || n0 >= g.n) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave >> 1) * WT, wn = (wave & 1) * WT;
     int klo = 0, khi = g.k;
-    if (g.a_lower && m0 + 64 < khi) khi = m0 + 64;
+    if (g.a_lower && m0 + T < khi) khi = m0 + T;
     if (g.b_lower) klo = n0;
     klo &= ~31;
-    v16f acc = {};
+    v16f acc[NB][NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[a][b] = v16f{};
     for (int k0 = klo; k0 < khi; k0 += 32) {
-        // stage A (64 x 32) and B (32 x 64)
-        for (int idx = tid; idx < 64 * 32; idx += 256) {
-            int mm = idx & 63, kk = idx >> 6;
+        for (int idx = tid; idx < T * 32; idx += 256) {
+            int mm = idx % T, kk = idx / T;
             int gm = m0 + mm, gk = k0 + kk;
-            // exact triangles: the strict upper parts of inverse blocks hold
-            // temporaries of the previous doubling level, so mask per element
             const bool aok = gm < g.m && gk < g.k && (!g.a_lower || gk <= gm);
             As[kk][mm] = aok ? A[gm + (int64_t)gk * g.lda] : 0.f;
             int k2 = idx & 31, nn = idx >> 5;
@@ -72,26 +79,35 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(gemm_batch g) {
         const int r = lane & 31, h = lane >> 5;
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
-            float a = As[2 * s + h][wm + r];
-            float b = Bs[2 * s + h][wn + r];
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+            float av[NB], bv[NB];
+#pragma unroll
+            for (int a = 0; a < NB; ++a) av[a] = As[2 * s + h][wm + 32 * a + r];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) bv[b] = Bs[2 * s + h][wn + 32 * b + r];
+#pragma unroll
+            for (int a = 0; a < NB; ++a)
+#pragma unroll
+                for (int b = 0; b < NB; ++b)
+                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[a], bv[b], acc[a][b], 0, 0, 0);
         }
         __syncthreads();
     }
     const int n = lane & 31, h = lane >> 5;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        int mm = m0 + wm + 8 * (i >> 2) + 4 * h + (i & 3);
-        int nn = n0 + wn + n;
-        if (mm < g.m && nn < g.n) C[mm + (int64_t)nn * g.ldc] = g.alpha * acc[i];
-    }
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                int mm = m0 + wm + 32 * a + 8 * (i >> 2) + 4 * h + (i & 3);
+                int nn = n0 + wn + 32 * b + n;
+                if (mm < g.m && nn < g.n) C[mm + (int64_t)nn * g.ldc] = g.alpha * acc[a][b][i];
+            }
 }
 
-// copy the 32x32 inverses onto the diagonal of the Winv blocks, zero the rest
-// of each 32-column strip's lower part handled later by the doubling
+// copy the 32x32 inverses onto the diagonal of the Winv blocks
 __global__ __launch_bounds__(256) void winv_init_kernel(const float* __restrict__ dinv, int64_t n,
                                                        int Bo, float* __restrict__ W) {
-    // one workgroup per 32-block b (global), writes the 32x32 diagonal tile
     const int64_t b = blockIdx.x;
     const int64_t r0 = b * 32;
     const int64_t I = r0 / Bo;
@@ -104,47 +120,68 @@ __global__ __launch_bounds__(256) void winv_init_kernel(const float* __restrict_
     }
 }
 
+// WT = Winv^T for each big block (lower part only is meaningful in Winv)
+__global__ __launch_bounds__(256) void transpose_blocks_kernel(const float* __restrict__ W,
+                                                              float* __restrict__ WT, int Bo) {
+    __shared__ float t[32][33];
+    const int64_t I = blockIdx.z;
+    const float* Wi = W + I * (int64_t)Bo * Bo;
+    float* Ti = WT + I * (int64_t)Bo * Bo;
+    const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+    if (r0 < c0) return;                              // upper tiles of Winv are unused
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int j = ty; j < 32; j += 8) {
+        float v = Wi[(r0 + tx) + (int64_t)(c0 + j) * Bo];
+        if (r0 == c0 && r0 + tx < c0 + j) v = 0.f;     // strict upper part of a diagonal tile
+        t[j][tx] = v;
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) Ti[(c0 + tx) + (int64_t)(r0 + j) * Bo] = t[tx][j];
+}
+
 // ---------------------------------------------------------------- TRSV kernels
-constexpr int NR = 8;   // right-hand sides per pass
+constexpr int NR = 4;   // right-hand sides per pass
+constexpr int UR = 16;  // loads in flight per lane
 
 // W[r][j] -= sum_{c < cw} L[r][c0 + c] * Y[c0 + c][j]  for r in [r0, n)
-// 64 rows per workgroup (lane = row), 4 waves split the columns.
-__global__ __launch_bounds__(256) void fwd_update_kernel(const float* __restrict__ L, int64_t ldl,
+// 64 rows per workgroup (lane = row, 256 B per column load), 8 waves split
+// the columns into contiguous ranges, UR independent loads per batch.
+__global__ __launch_bounds__(512) void fwd_update_kernel(const float* __restrict__ L, int64_t ldl,
                                                         int64_t n, int64_t r0, int64_t c0, int cw,
                                                         const double* __restrict__ Y, int64_t ldy,
                                                         double* __restrict__ W, int64_t ldw,
                                                         int nrhs) {
-    __shared__ double red[4][NR][64];
+    __shared__ double red[8][NR][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t r = r0 + (int64_t)blockIdx.x * 64 + lane;
     const int j0 = blockIdx.y * NR;
     const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
+    const int per = (cw + 7) / 8;
+    const int cb = wave * per, ce = cb + per < cw ? cb + per : cw;
     double acc[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-    if (r < n) {
-        const float* Lr = L + r + c0 * ldl;
-        const double* Yc = Y + c0 + (int64_t)j0 * ldy;
-        int c = wave;
-        for (; c + 12 < cw; c += 16) {
-            float l0 = Lr[(int64_t)c * ldl], l1 = Lr[(int64_t)(c + 4) * ldl];
-            float l2 = Lr[(int64_t)(c + 8) * ldl], l3 = Lr[(int64_t)(c + 12) * ldl];
+    const int64_t rr = r < n ? r : n - 1;          // clamp: loads stay in bounds
+    const float* Lr = L + rr + c0 * ldl;
+    const double* Yc = Y + c0 + (int64_t)j0 * ldy;
+    int c = cb;
+    for (; c + UR <= ce; c += UR) {
+        float l[UR];
 #pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (j < nj) {
-                    const double* y = Yc + (int64_t)j * ldy;
-                    acc[j] = fma((double)l0, y[c], acc[j]);
-                    acc[j] = fma((double)l1, y[c + 4], acc[j]);
-                    acc[j] = fma((double)l2, y[c + 8], acc[j]);
-                    acc[j] = fma((double)l3, y[c + 12], acc[j]);
-                }
-        }
-        for (; c < cw; c += 4) {
-            float l0 = Lr[(int64_t)c * ldl];
+        for (int u = 0; u < UR; ++u) l[u] = Lr[(int64_t)(c + u) * ldl];
 #pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (j < nj) acc[j] = fma((double)l0, Yc[(int64_t)j * ldy + c], acc[j]);
-        }
+        for (int j = 0; j < NR; ++j)
+            if (j < nj) {
+                const double* y = Yc + (int64_t)j * ldy + c;
+#pragma unroll
+                for (int u = 0; u < UR; ++u) acc[j] = fma((double)l[u], y[u], acc[j]);
+            }
+    }
+    for (; c < ce; ++c) {
+        float l0 = Lr[(int64_t)c * ldl];
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+            if (j < nj) acc[j] = fma((double)l0, Yc[(int64_t)j * ldy + c], acc[j]);
     }
 #pragma unroll
     for (int j = 0; j < NR; ++j) red[wave][j][lane] = acc[j];
@@ -153,48 +190,96 @@ __global__ __launch_bounds__(256) void fwd_update_kernel(const float* __restrict
 #pragma unroll
         for (int j = 0; j < NR; ++j)
             if (j < nj) {
-                double s = (red[0][j][lane] + red[1][j][lane]) + (red[2][j][lane] + red[3][j][lane]);
+                double s = 0.0;
+#pragma unroll
+                for (int w = 0; w < 8; ++w) s += red[w][j][lane];
                 W[r + (int64_t)(j0 + j) * ldw] -= s;
             }
     }
 }
 
-// Y[p0 + r][j] = sum_{c <= r} Winv[r][c] W[p0 + c][j]  (rows of one big block)
-__global__ __launch_bounds__(256) void fwd_diag_kernel(const float* __restrict__ Wi, int Bo, int bI,
-                                                      int64_t p0, const double* __restrict__ W,
-                                                      int64_t ldw, double* __restrict__ Y,
-                                                      int64_t ldy, int nrhs) {
-    __shared__ double red[4][NR][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r = blockIdx.x * 64 + lane;
-    const int j0 = blockIdx.y * NR;
-    const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
+// wave-per-vector dot of a contiguous float segment with FP64 vectors:
+// out[j] = sum_{t < len} seg[t] * V[off + t + j*ldv]
+__device__ __forceinline__ void seg_dot(const float* __restrict__ seg, int len,
+                                        const double* __restrict__ V, int64_t off, int64_t ldv,
+                                        int nj, double out[NR]) {
+    const int lane = threadIdx.x & 63;
     double acc[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-    if (r < bI) {
-        const double* Wc = W + p0 + (int64_t)j0 * ldw;
-        for (int c = wave; c <= r; c += 4) {
-            float w = Wi[r + (int64_t)c * Bo];
+    int t = lane;
+    for (; t + 64 * (UR / 2 - 1) < len; t += 64 * (UR / 2)) {
+        float s[UR / 2];
 #pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (j < nj) acc[j] = fma((double)w, Wc[(int64_t)j * ldw + c], acc[j]);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < NR; ++j) red[wave][j][lane] = acc[j];
-    __syncthreads();
-    if (wave == 0 && r < bI) {
+        for (int u = 0; u < UR / 2; ++u) s[u] = seg[t + 64 * u];
 #pragma unroll
         for (int j = 0; j < NR; ++j)
-            if (j < nj)
-                Y[p0 + r + (int64_t)(j0 + j) * ldy] =
-                    (red[0][j][lane] + red[1][j][lane]) + (red[2][j][lane] + red[3][j][lane]);
+            if (j < nj) {
+                const double* v = V + off + (int64_t)j * ldv + t;
+#pragma unroll
+                for (int u = 0; u < UR / 2; ++u) acc[j] = fma((double)s[u], v[64 * u], acc[j]);
+            }
+    }
+    for (; t < len; t += 64) {
+        float s0 = seg[t];
+#pragma unroll
+        for (int j = 0; j < NR; ++j)
+            if (j < nj) acc[j] = fma((double)s0, V[off + (int64_t)j * ldv + t], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        double s = acc[j];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
+        out[j] = s;
+    }
+}
+
+// Y[p0 + r][j] = sum_{c <= r} Winv[r][c] W[p0 + c][j]: row r of Winv is column
+// r of WT (contiguous).  One wave per row, 4 rows per wave.
+__global__ __launch_bounds__(256) void fwd_diag_kernel(const float* __restrict__ WTi, int Bo, int bI,
+                                                      int64_t p0, const double* __restrict__ W,
+                                                      int64_t ldw, double* __restrict__ Y,
+                                                      int64_t ldy, int nrhs) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j0 = blockIdx.y * NR;
+    const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
+    for (int q = 0; q < 4; ++q) {
+        const int r = blockIdx.x * 16 + wave * 4 + q;
+        if (r >= bI) break;
+        double out[NR];
+        seg_dot(WTi + (int64_t)r * Bo, r + 1, W, p0 + (int64_t)j0 * ldw, ldw, nj, out);
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j)
+                if (j < nj) Y[p0 + r + (int64_t)(j0 + j) * ldy] = out[j];
+        }
+    }
+}
+
+// X[p0 + c][j] = sum_{r >= c} Winv[r][c] W[p0 + r][j]: column c of Winv
+__global__ __launch_bounds__(256) void bwd_diag_kernel(const float* __restrict__ Wi, int Bo, int bI,
+                                                      int64_t p0, const double* __restrict__ W,
+                                                      int64_t ldw, double* __restrict__ X,
+                                                      int64_t ldx, int nrhs) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int j0 = blockIdx.y * NR;
+    const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
+    for (int q = 0; q < 4; ++q) {
+        const int c = blockIdx.x * 16 + wave * 4 + q;
+        if (c >= bI) break;
+        double out[NR];
+        seg_dot(Wi + c + (int64_t)c * Bo, bI - c, W, p0 + c + (int64_t)j0 * ldw, ldw, nj, out);
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j)
+                if (j < nj) X[p0 + c + (int64_t)(j0 + j) * ldx] = out[j];
+        }
     }
 }
 
 // W[c][j] -= sum_{r < bI} L[p0 + r][c] * X[p0 + r][j]   for c < p0
-// one wave per column, 4 columns per wave, 16 per workgroup
+// one wave per column (contiguous 4*bI bytes), 4 columns per wave
 __global__ __launch_bounds__(256) void bwd_update_kernel(const float* __restrict__ L, int64_t ldl,
                                                         int64_t p0, int bI,
                                                         const double* __restrict__ X, int64_t ldx,
@@ -203,68 +288,15 @@ __global__ __launch_bounds__(256) void bwd_update_kernel(const float* __restrict
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int j0 = blockIdx.y * NR;
     const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
-    const int64_t cbase = (int64_t)blockIdx.x * 16 + wave * 4;
     for (int q = 0; q < 4; ++q) {
-        const int64_t c = cbase + q;
+        const int64_t c = (int64_t)blockIdx.x * 16 + wave * 4 + q;
         if (c >= p0) break;
-        const float* col = L + p0 + c * ldl;
-        double acc[NR];
-#pragma unroll
-        for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-        for (int r = lane; r < bI; r += 64) {
-            float l = col[r];
-#pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (j < nj) acc[j] = fma((double)l, X[p0 + r + (int64_t)(j0 + j) * ldx], acc[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            double s = acc[j];
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
-            acc[j] = s;
-        }
+        double out[NR];
+        seg_dot(L + p0 + c * ldl, bI, X, p0 + (int64_t)j0 * ldx, ldx, nj, out);
         if (lane == 0) {
 #pragma unroll
             for (int j = 0; j < NR; ++j)
-                if (j < nj) W[c + (int64_t)(j0 + j) * ldw] -= acc[j];
-        }
-    }
-}
-
-// X[p0 + c][j] = sum_{r >= c} Winv[r][c] W[p0 + r][j]
-__global__ __launch_bounds__(256) void bwd_diag_kernel(const float* __restrict__ Wi, int Bo, int bI,
-                                                      int64_t p0, const double* __restrict__ W,
-                                                      int64_t ldw, double* __restrict__ X,
-                                                      int64_t ldx, int nrhs) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j0 = blockIdx.y * NR;
-    const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
-    const int cbase = blockIdx.x * 16 + wave * 4;
-    for (int q = 0; q < 4; ++q) {
-        const int c = cbase + q;
-        if (c >= bI) break;
-        const float* col = Wi + (int64_t)c * Bo;
-        double acc[NR];
-#pragma unroll
-        for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-        for (int r = c + lane; r < bI; r += 64) {
-            float w = col[r];
-#pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (j < nj) acc[j] = fma((double)w, W[p0 + r + (int64_t)(j0 + j) * ldw], acc[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            double s = acc[j];
-#pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
-            acc[j] = s;
-        }
-        if (lane == 0) {
-#pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (j < nj) X[p0 + c + (int64_t)(j0 + j) * ldx] = acc[j];
+                if (j < nj) W[c + (int64_t)(j0 + j) * ldw] -= out[j];
         }
     }
 }
@@ -272,8 +304,13 @@ __global__ __launch_bounds__(256) void bwd_diag_kernel(const float* __restrict__
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 int launch_gemm(hipStream_t st, const gemm_batch& g, int nbatch) {
-    dim3 grid(cdiv(g.m, 64), cdiv(g.n, 64), (unsigned)nbatch);
-    hipLaunchKernelGGL(gemm_f32_kernel, grid, dim3(256), 0, st, g);
+    if (g.m >= 128 && g.n >= 128) {
+        dim3 grid(cdiv(g.m, 128), cdiv(g.n, 128), (unsigned)nbatch);
+        hipLaunchKernelGGL(gemm_f32_kernel<128>, grid, dim3(256), 0, st, g);
+    } else {
+        dim3 grid(cdiv(g.m, 64), cdiv(g.n, 64), (unsigned)nbatch);
+        hipLaunchKernelGGL(gemm_f32_kernel<64>, grid, dim3(256), 0, st, g);
+    }
     MPC_LAUNCH_CHECK();
     return 0;
 }
@@ -281,82 +318,93 @@ int launch_gemm(hipStream_t st, const gemm_batch& g, int nbatch) {
 }  // namespace
 
 // Build Winv_I = L_II^{-1} for every big block (Bo x Bo, column-major, ld Bo,
-// block I at W + I*Bo*Bo) from L and its 32x32 diagonal inverses.
+// block I at W + I*Bo*Bo) from L and its 32x32 diagonal inverses, then
+// WT_I = Winv_I^T.  Full big blocks are batched per level; a ragged last big
+// block is handled by its own launches with clipped sizes.
+static int double_level(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo, float* W,
+                        int h, int64_t I0, int nblk, int64_t rows, int q0, int nq, int hc) {
+    const int64_t base = I0 * Bo;
+    const int64_t p = 2 * (int64_t)h * q0;
+    float* Wi = W + I0 * (int64_t)Bo * Bo;
+    const int64_t bs = (int64_t)Bo * Bo;                 // stride between big blocks in W
+    gemm_batch g1;                                       // T = B * Ainv -> upper part (p, p+h)
+    g1.A = L + (base + p + h) + (base + p) * ldl;  g1.lda = ldl;
+    g1.B = Wi + p + p * (int64_t)Bo;              g1.ldb = Bo;
+    g1.C = Wi + p + (p + h) * (int64_t)Bo;        g1.ldc = Bo;
+    g1.sA = 2 * (int64_t)h * (1 + ldl);
+    g1.sB = g1.sC = 2 * (int64_t)h * (1 + Bo);
+    g1.tA = (int64_t)Bo * (1 + ldl);
+    g1.tB = g1.tC = bs;
+    g1.nq = nq;
+    g1.m = hc; g1.n = h; g1.k = h; g1.alpha = 1.f; g1.a_lower = 0; g1.b_lower = 1;
+    int rc = launch_gemm(st, g1, nq * nblk);
+    if (rc) return rc;
+    gemm_batch g2;                                       // X21 = -Cinv * T -> (p+h, p)
+    g2.A = Wi + (p + h) + (p + h) * (int64_t)Bo;  g2.lda = Bo;
+    g2.B = Wi + p + (p + h) * (int64_t)Bo;        g2.ldb = Bo;
+    g2.C = Wi + (p + h) + p * (int64_t)Bo;        g2.ldc = Bo;
+    g2.sA = g2.sB = g2.sC = 2 * (int64_t)h * (1 + Bo);
+    g2.tA = g2.tB = g2.tC = bs;
+    g2.nq = nq;
+    g2.m = hc; g2.n = h; g2.k = hc; g2.alpha = -1.f; g2.a_lower = 1; g2.b_lower = 0;
+    (void)rows;
+    return launch_gemm(st, g2, nq * nblk);
+}
+
 int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
-                             const float* dinv, float* W) {
+                             const float* dinv, float* W, float* WT) {
     if (Bo % 64 || Bo < 64) {
         mpc_set_error("block inverse: Bo must be a multiple of 64");
         return MPC_ERR_ARG;
     }
     const int64_t nI = (n + Bo - 1) / Bo;
+    const int64_t nfull = n / Bo;                 // big blocks with Bo rows
     const int64_t nb32 = (n + 31) / 32;
     hipLaunchKernelGGL(winv_init_kernel, dim3((unsigned)nb32), dim3(256), 0, st, dinv, n, Bo, W);
     MPC_LAUNCH_CHECK();
-    // doubling: h = 32, 64, ..., Bo/2.  Pairs of h-blocks p = 2h q inside each
-    // big block.  Rows beyond n (ragged last big block) are treated as part of
-    // an identity extension: their L entries are never read (m, n, k clipped).
+    int rc;
     for (int h = 32; h < Bo; h *= 2) {
         const int pairs = Bo / (2 * h);
-        for (int64_t I = 0; I < nI; ++I) {
-            const int64_t base = I * Bo;
-            const int64_t rows = n - base < Bo ? n - base : Bo;
+        if (nfull > 0 && (rc = double_level(st, L, ldl, n, Bo, W, h, 0, (int)nfull, Bo, 0, pairs, h)))
+            return rc;
+        if (nI > nfull) {                          // ragged last big block
+            const int64_t I = nfull, rows = n - I * Bo;
             int npair = 0;
             for (int q = 0; q < pairs; ++q)
                 if (2 * h * q + h < rows) npair = q + 1;
             if (npair == 0) continue;
-            // all pairs of this big block in one batched launch; the last pair
-            // may be ragged, so clip its sizes separately
             const int full = (2 * h * (npair - 1) + 2 * h <= rows) ? npair : npair - 1;
-            float* Wi = W + I * (int64_t)Bo * Bo;
-            for (int part = 0; part < 2; ++part) {
-                const int q0 = part == 0 ? 0 : full;
-                const int qn = part == 0 ? full : npair - full;
-                if (qn <= 0) continue;
-                const int64_t p = 2 * (int64_t)h * q0;
-                const int hc = part == 0 ? h : (int)(rows - p - h);   // C-block size
-                // T = B * Ainv  (T stored in the upper triangle: position (p, p + h))
-                gemm_batch g1;
-                g1.A = L + (base + p + h) + (base + p) * ldl;  g1.lda = ldl;
-                g1.B = Wi + p + p * (int64_t)Bo;              g1.ldb = Bo;
-                g1.C = Wi + p + (p + h) * (int64_t)Bo;        g1.ldc = Bo;
-                g1.sA = 2 * (int64_t)h * (1 + ldl);
-                g1.sB = 2 * (int64_t)h * (1 + Bo);
-                g1.sC = 2 * (int64_t)h * (1 + Bo);
-                g1.m = hc; g1.n = h; g1.k = h; g1.alpha = 1.f; g1.a_lower = 0; g1.b_lower = 1;
-                int rc = launch_gemm(st, g1, qn);
-                if (rc) return rc;
-                // X21 = -Cinv * T  -> position (p + h, p)
-                gemm_batch g2;
-                g2.A = Wi + (p + h) + (p + h) * (int64_t)Bo;  g2.lda = Bo;
-                g2.B = Wi + p + (p + h) * (int64_t)Bo;        g2.ldb = Bo;
-                g2.C = Wi + (p + h) + p * (int64_t)Bo;        g2.ldc = Bo;
-                g2.sA = g2.sB = g2.sC = 2 * (int64_t)h * (1 + Bo);
-                g2.m = hc; g2.n = h; g2.k = hc; g2.alpha = -1.f; g2.a_lower = 1; g2.b_lower = 0;
-                rc = launch_gemm(st, g2, qn);
-                if (rc) return rc;
+            if (full > 0 && (rc = double_level(st, L, ldl, n, Bo, W, h, I, 1, rows, 0, full, h)))
+                return rc;
+            if (npair > full) {
+                const int hc = (int)(rows - 2 * (int64_t)h * full - h);
+                if ((rc = double_level(st, L, ldl, n, Bo, W, h, I, 1, rows, full, 1, hc))) return rc;
             }
         }
     }
+    dim3 tg((unsigned)(Bo / 32), (unsigned)(Bo / 32), (unsigned)nI);
+    hipLaunchKernelGGL(transpose_blocks_kernel, tg, dim3(256), 0, st, W, WT, Bo);
+    MPC_LAUNCH_CHECK();
     return 0;
 }
 
 // z = L^{-T} L^{-1} w for nrhs columns.  W (in/out, destroyed), Y (work), Z (out)
 int mpc_apply_block_prec(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
-                         const float* Winv, double* W, double* Y, double* Z, int64_t ld, int nrhs) {
+                         const float* Winv, const float* WinvT, double* W, double* Y, double* Z,
+                         int64_t ld, int nrhs) {
     const int64_t nI = (n + Bo - 1) / Bo;
     const unsigned gz = cdiv(nrhs, NR);
     for (int64_t I = 0; I < nI; ++I) {
         const int64_t p0 = I * Bo;
         const int bI = (int)(n - p0 < Bo ? n - p0 : Bo);
-        const float* Wi = Winv + I * (int64_t)Bo * Bo;
-        hipLaunchKernelGGL(fwd_diag_kernel, dim3(cdiv(bI, 64), gz), dim3(256), 0, st, Wi, Bo, bI,
+        const float* WTi = WinvT + I * (int64_t)Bo * Bo;
+        hipLaunchKernelGGL(fwd_diag_kernel, dim3(cdiv(bI, 16), gz), dim3(256), 0, st, WTi, Bo, bI,
                            p0, W, ld, Y, ld, nrhs);
         const int64_t r0 = p0 + bI;
         if (r0 < n)
-            hipLaunchKernelGGL(fwd_update_kernel, dim3(cdiv(n - r0, 64), gz), dim3(256), 0, st, L,
+            hipLaunchKernelGGL(fwd_update_kernel, dim3(cdiv(n - r0, 64), gz), dim3(512), 0, st, L,
                                ldl, n, r0, p0, bI, Y, ld, W, ld, nrhs);
     }
-    // backward: Y is the right-hand side now (destroyed), Z the solution
     for (int64_t I = nI - 1; I >= 0; --I) {
         const int64_t p0 = I * Bo;
         const int bI = (int)(n - p0 < Bo ? n - p0 : Bo);

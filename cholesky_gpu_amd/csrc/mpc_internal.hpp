@@ -62,6 +62,12 @@ struct mpc_handle {
     int          device = 0;
     hipStream_t  stream = nullptr;
     bool         own_stream = false;
+    // look-ahead: the next panel (POTRF, TRSM, quantise) runs on pstream while
+    // the trailing update of the current panel runs on stream; both at the
+    // default priority
+    hipStream_t  pstream = nullptr;
+    hipEvent_t   ev_col = nullptr;     // next block column updated (on stream)
+    hipEvent_t   ev_panel = nullptr;   // next panel ready (on pstream)
     // factor workspace
     mpc_workspace panel;      // quantised panel payload, n * nb * 4 bytes
     mpc_workspace scales;     // n * nb/32 bytes
@@ -93,41 +99,50 @@ struct mpc_small {
     int      nan_flag;        // NaN/Inf seen in a panel
     uint32_t mu_bits;         // min pivot^2 so far (bits of a positive float; atomicMin)
     uint32_t pad;
-    // per-step tile stats (index by tile within the panel), max 1024 tiles
-    uint32_t tile_amax[1024];   // bits of positive floats (atomicMax)
-    uint32_t tile_maxrs[1024];  // max row sum of squares
-    uint8_t  tile_prec[1024];   // decided precision per panel tile
+    // per-step tile stats (index by tile within the panel), max 1024 tiles,
+    // double-buffered by step parity so panel j+1 can be prepared while the
+    // trailing update of panel j still reads panel j's buffers
+    uint32_t tile_amax[2][1024];   // bits of positive floats (atomicMax)
+    uint32_t tile_maxrs[2][1024];  // max row sum of squares
+    uint8_t  tile_prec[2][1024];   // decided precision per panel tile
 };
 
 // ---------------------------------------------------------------- kernels (launchers)
 // k_potrf.hip
 int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0,
-                     float* dinv, mpc_small* sm, int ntiles_next);
+                     float* dinv, mpc_small* sm, int ntiles_next, uint32_t* stats_amax,
+                     uint32_t* stats_maxrs);
 // k_panel.hip
 int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, int nb,
                           const float* dinv_j, const float* Ljj, float* rowss, float* tracker,
-                          int tile_rows, mpc_small* sm);
+                          int tile_rows, mpc_small* sm, uint32_t* tile_amax, uint32_t* tile_maxrs);
 int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
                            int tile_rows, const uint8_t* prec_in, const uint32_t* tile_amax,
-                           mpc_small* sm, int update_prec, const int* ladder, int n_ladder,
-                           double eps_prime, uint8_t* prec_out, uint8_t* map_col,
-                           uint8_t* payload, int64_t row_stride, uint8_t* scales, int* nan_flag);
+                           const uint32_t* tile_maxrs, mpc_small* sm, int update_prec,
+                           const int* ladder, int n_ladder, double eps_prime, uint8_t* prec_out,
+                           uint8_t* map_col, uint8_t* payload, int64_t row_stride, uint8_t* scales,
+                           int* nan_flag);
 int mpc_launch_tile_amax(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
                          int tile_rows, uint32_t* tile_amax);
 // k_syrk.hip
 int mpc_launch_syrk(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                     const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                     int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm);
+constexpr int MPC_SYRK_TILE = 128;   // output tile edge of the SYRK kernels
 int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                             const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                             int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
-                            unsigned class_mask, unsigned fp8_mask);
+                            unsigned class_mask, unsigned fp8_mask, int64_t tcol0 = 0,
+                            int64_t tcol1 = -1);
 // compute classes / FP8 element formats that a factor configuration can produce
 void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* class_mask,
                     unsigned* fp8_mask);
 // k_pcg.hip
 int mpc_launch_gemv(hipStream_t st, const float* A, int64_t lda, int64_t n, int nrhs,
                     const double* X, int64_t ldx, double* Y, int64_t ldy, double* part);
+int mpc_launch_symv(hipStream_t st, const float* A, int64_t lda, int64_t n, int nrhs,
+                    const double* X, int64_t ldx, double* Y, int64_t ldy, double* part);
+int64_t mpc_symv_partial_doubles(int64_t n, int nrhs);
 int mpc_launch_trsv_step_fwd(hipStream_t st, const float* L, int64_t ldl, int64_t n, int nb,
                              int j, const float* dinv, double* W, int64_t ldw, int nrhs,
                              double* Y, int64_t ldy);
@@ -162,9 +177,10 @@ int mpc_launch_dinv_from_L(hipStream_t st, const float* L, int64_t ldl, int64_t 
 int mpc_gemv_nsplit(int64_t n);
 // k_tri.hip
 int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
-                             const float* dinv, float* W);
+                             const float* dinv, float* W, float* WT);
 int mpc_apply_block_prec(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
-                         const float* Winv, double* W, double* Y, double* Z, int64_t ld, int nrhs);
+                         const float* Winv, const float* WinvT, double* W, double* Y, double* Z,
+                         int64_t ld, int nrhs);
 inline int mpc_block_bo(int64_t n) { return n > 8192 ? 2048 : (n > 2048 ? 1024 : 256); }
 
 // host-side generators shared with mpc_matgen (RNG identical to the device one)

@@ -162,7 +162,7 @@ def test_syrk_classes(handle, oracle, pa, pb):
     print("syrk max rel err", L.PREC_NAMES[pa], L.PREC_NAMES[pb], errs, flush=True)
     # the FP8 matrix-core path accumulates with slightly looser rounding than a
     # sequential FP32 sum (measured ~2.4e-5 of sum |terms| at k = 256)
-    tol = 1e-4 if min(pa, pb) >= L.PREC_E5M2 else 2e-5
+    tol = 1e-4 if max(pa, pb) >= L.PREC_E5M2 else 2e-5
     assert errs["keep"] < tol, errs
     assert np.array_equal(np.triu(got, 1), np.triu(Cm, 1))   # upper untouched
 
