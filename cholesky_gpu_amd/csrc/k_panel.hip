@@ -24,8 +24,20 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ int sigma(int s, int h) { return 8 * (s >> 2) + 4 * h + (s & 3); }
 
-template <int NBK>
-__global__ __launch_bounds__(256) void trsm_stats_kernel(float* __restrict__ A21, int64_t lda,
+// one refinement step after the product with the explicit 32x32 inverse
+constexpr bool kTrsmRefine = true;
+
+// One workgroup of NW waves per 32-row slab.  Wave w owns the column blocks
+// b = w, w + NW, ...  Step b: the owner finishes X_b = acc_b Dinv_b^T (one
+// refinement step against L_bb when REF), publishes it in LDS; after the
+// barrier every wave subtracts X_b L_{b'b}^T from its owned blocks b' > b,
+// the soonest-needed block first.  The L operands for a step are loaded
+// before the barrier so their latency overlaps the owner's solve.
+// MFMA roles (32x32x2 f32, k index sig(t, h) on both operands): A = L or
+// Dinv rows (lane m = lane % 32), B = X or acc columns (lane n = row of the
+// slab); accumulator register i of lane (n, h) holds column sig(i, h), row n.
+template <int NBK, int NW, bool REF>
+__global__ __launch_bounds__(NW * 64) void trsm_stats_kernel(float* __restrict__ A21, int64_t lda,
                                                          int64_t s, const float* __restrict__ dinv_j,
                                                          const float* __restrict__ Ljj,
                                                          float* __restrict__ rowss,
@@ -33,84 +45,112 @@ __global__ __launch_bounds__(256) void trsm_stats_kernel(float* __restrict__ A21
                                                          mpc_small* __restrict__ sm,
                                                          uint32_t* __restrict__ t_amax,
                                                          uint32_t* __restrict__ t_maxrs) {
+    constexpr int NO = NBK / NW;          // blocks per wave
+    constexpr int XLD = 33;
+    __shared__ float Xs[NBK][32 * XLD];
+    __shared__ float red_ss[NW][64], red_am[NW][64];
     if (sm->info != 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t r0 = ((int64_t)blockIdx.x * 4 + wave) * 32;
-    if (r0 >= s) return;
     const int n = lane & 31, h = lane >> 5;
+    const int64_t r0 = (int64_t)blockIdx.x * 32;
     const int64_t row = r0 + n;
     const bool valid = row < s;
-    v16f X[NBK];
+
+    v16f acc[NO];
 #pragma unroll
-    for (int b = 0; b < NBK; ++b) {
-        v16f acc;
+    for (int o = 0; o < NO; ++o) {
+        const int b = wave + NW * o;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            int q = sigma(i, h);
-            acc[i] = valid ? A21[row + (int64_t)(32 * b + q) * lda] : 0.f;
-        }
-#pragma unroll
-        for (int p = 0; p < b; ++p) {
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                float a = -Ljj[(32 * b + n) + (int64_t)(32 * p + sigma(t, h)) * lda];
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, X[p][t], acc, 0, 0, 0);
-            }
-        }
-        v16f xb = {};
-        const float* Db = dinv_j + b * 1024;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            float a = Db[n + sigma(t, h) * 32];
-            xb = __builtin_amdgcn_mfma_f32_32x32x2f32(a, acc[t], xb, 0, 0, 0);
-        }
-        // one refinement step, r = acc - xb L_bb^T, xb += r Dinv_b^T: the
-        // product with the explicit inverse alone carries an error that grows
-        // with cond(L_bb); after one correction the result matches
-        // substitution (the strict upper part of L_jj holds A and is masked)
-        v16f rr = acc;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            const int q = sigma(t, h);
-            float a = q <= n ? -Ljj[(32 * b + n) + (int64_t)(32 * b + q) * lda] : 0.f;
-            rr = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xb[t], rr, 0, 0, 0);
-        }
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            float a = Db[n + sigma(t, h) * 32];
-            xb = __builtin_amdgcn_mfma_f32_32x32x2f32(a, rr[t], xb, 0, 0, 0);
-        }
-        X[b] = xb;
-        if (valid) {
-#pragma unroll
-            for (int i = 0; i < 16; ++i) A21[row + (int64_t)(32 * b + sigma(i, h)) * lda] = xb[i];
-        }
+        for (int i = 0; i < 16; ++i)
+            acc[o][i] = valid ? A21[row + (int64_t)(32 * b + sigma(i, h)) * lda] : 0.f;
     }
     float ss = 0.f, am = 0.f;
 #pragma unroll
-    for (int b = 0; b < NBK; ++b)
+    for (int b = 0; b < NBK; ++b) {
+        // prefetch this step's L operands: -L(32 b' + m, 32 b + sig(t, h)) for owned b' > b
+        float lv[NO][16];
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            ss = fmaf(X[b][i], X[b][i], ss);
-            am = fmaxf(am, fabsf(X[b][i]));
+        for (int o = 0; o < NO; ++o) {
+            const int bp = wave + NW * o;
+#pragma unroll
+            for (int t = 0; t < 16; ++t)
+                lv[o][t] = bp > b ? -Ljj[(32 * bp + n) + (int64_t)(32 * b + sigma(t, h)) * lda] : 0.f;
         }
-    ss += __shfl_xor(ss, 32);
-    am = fmaxf(am, __shfl_xor(am, 32));
-    if (!valid) { ss = 0.f; am = 0.f; }
-    if (valid && h == 0) {
-        rowss[row] = ss;
-        tracker[row] -= ss;
-    }
-    if (!(ss == ss) || !(am <= 3.4e38f)) atomicOr(&sm->nan_flag, 1);
-    // wave max (all 32 rows of the slab are in one tile: tile_rows % 32 == 0)
-    float wss = ss, wam = am;
+        if (b % NW == wave) {
+            const int o = b / NW;
+            const float* Db = dinv_j + b * 1024;
+            v16f xb = {};
 #pragma unroll
-    for (int o = 16; o >= 1; o >>= 1) {
+            for (int t = 0; t < 16; ++t)
+                xb = __builtin_amdgcn_mfma_f32_32x32x2f32(Db[n + sigma(t, h) * 32], acc[o][t], xb, 0,
+                                                          0, 0);
+            if constexpr (REF) {
+                // r = acc - xb L_bb^T (strict upper part of L_jj holds A: masked), xb += r Dinv^T
+                v16f rr = acc[o];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) {
+                    const int q = sigma(t, h);
+                    const float a = q <= n ? -Ljj[(32 * b + n) + (int64_t)(32 * b + q) * lda] : 0.f;
+                    rr = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xb[t], rr, 0, 0, 0);
+                }
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+                    xb = __builtin_amdgcn_mfma_f32_32x32x2f32(Db[n + sigma(t, h) * 32], rr[t], xb, 0,
+                                                              0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                Xs[b][n + sigma(i, h) * XLD] = xb[i];
+                ss = fmaf(xb[i], xb[i], ss);
+                am = fmaxf(am, fabsf(xb[i]));
+            }
+            if (valid) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) A21[row + (int64_t)(32 * b + sigma(i, h)) * lda] = xb[i];
+            }
+        }
+        if (b + 1 < NBK) {
+            __syncthreads();
+#pragma unroll
+            for (int o = 0; o < NO; ++o) {
+                const int bp = wave + NW * o;
+                if (bp > b) {
+#pragma unroll
+                    for (int t = 0; t < 16; ++t)
+                        acc[o] = __builtin_amdgcn_mfma_f32_32x32x2f32(
+                            lv[o][t], Xs[b][n + sigma(t, h) * XLD], acc[o], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // row statistics: lane (n, h) of every wave holds part of row n
+    red_ss[wave][lane] = ss;
+    red_am[wave][lane] = am;
+    __syncthreads();
+    if (wave != 0) return;
+    float tss = 0.f, tam = 0.f;
+    if (lane < 32) {
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            tss += red_ss[w][lane] + red_ss[w][lane + 32];
+            tam = fmaxf(tam, fmaxf(red_am[w][lane], red_am[w][lane + 32]));
+        }
+    }
+    if (!valid || lane >= 32) { tss = 0.f; tam = 0.f; }
+    if (valid && lane < 32) {
+        rowss[row] = tss;
+        tracker[row] -= tss;
+    }
+    if (!(tss == tss) || !(tam <= 3.4e38f)) atomicOr(&sm->nan_flag, 1);
+    // slab max (all 32 rows of the slab are in one tile: tile_rows % 32 == 0)
+    float wss = tss, wam = tam;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
         wss = fmaxf(wss, __shfl_xor(wss, o));
         wam = fmaxf(wam, __shfl_xor(wam, o));
     }
     if (lane == 0) {
-        int t = (int)(r0 / tile_rows);
+        const int t = (int)(r0 / tile_rows);
         atomicMax(&t_maxrs[t], __float_as_uint(wss));
         atomicMax(&t_amax[t], __float_as_uint(wam));
     }
@@ -236,19 +276,22 @@ int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, in
         mpc_set_error("trsm: tile_rows %d not a multiple of 32", tile_rows);
         return MPC_ERR_ARG;
     }
-    dim3 grid((unsigned)((s + 127) / 128));
+    dim3 grid((unsigned)((s + 31) / 32));
     switch (nb) {
     case 64:
-        hipLaunchKernelGGL(trsm_stats_kernel<2>, grid, dim3(256), 0, st, A21, lda, s, dinv_j, Ljj,
-                           rowss, tracker, tile_rows, sm, tile_amax, tile_maxrs);
+        hipLaunchKernelGGL((trsm_stats_kernel<2, 2, kTrsmRefine>), grid, dim3(128), 0, st, A21,
+                           lda, s, dinv_j, Ljj, rowss, tracker, tile_rows, sm, tile_amax,
+                           tile_maxrs);
         break;
     case 128:
-        hipLaunchKernelGGL(trsm_stats_kernel<4>, grid, dim3(256), 0, st, A21, lda, s, dinv_j, Ljj,
-                           rowss, tracker, tile_rows, sm, tile_amax, tile_maxrs);
+        hipLaunchKernelGGL((trsm_stats_kernel<4, 4, kTrsmRefine>), grid, dim3(256), 0, st, A21,
+                           lda, s, dinv_j, Ljj, rowss, tracker, tile_rows, sm, tile_amax,
+                           tile_maxrs);
         break;
     case 256:
-        hipLaunchKernelGGL(trsm_stats_kernel<8>, grid, dim3(256), 0, st, A21, lda, s, dinv_j, Ljj,
-                           rowss, tracker, tile_rows, sm, tile_amax, tile_maxrs);
+        hipLaunchKernelGGL((trsm_stats_kernel<8, 4, kTrsmRefine>), grid, dim3(256), 0, st, A21,
+                           lda, s, dinv_j, Ljj, rowss, tracker, tile_rows, sm, tile_amax,
+                           tile_maxrs);
         break;
     default:
         mpc_set_error("trsm: unsupported block size %d", nb);

@@ -5,8 +5,9 @@
 // micro_cholesky, micro_chol.hpp:14-44).  The block lives in LDS as 32x32
 // sub-tiles (lower triangle only, 36 sub-tiles x 32x33 floats = 152 KB for
 // bs = 256); the algorithm is right-looking over 32-column sub-panels:
-//   1. wave 0 factors the diagonal sub-tile in registers (row per lane) and
-//      inverts it (column per lane);
+//   1. wave 0 factors the diagonal sub-tile and inverts it in one
+//      right-looking pass (row per lane, pivot column and inverse row
+//      broadcast through LDS);
 //   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32)
 //      with one refinement step against the factored sub-tile;
 //   3. the trailing sub-tiles receive the rank-32 update (MFMA 32x32x2 f32).
@@ -56,6 +57,8 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
                                                     uint32_t* __restrict__ st_maxrs) {
     __shared__ float T[NSUB * SUBSZ];
     __shared__ float Dinv[SUBSZ];
+    __shared__ float colb[2][32];                   // column k of L, broadcast
+    __shared__ __align__(16) float xb[2][32];       // row k of L^{-1}, broadcast
     __shared__ int fail_col;
     __shared__ float red[256];
 
@@ -68,10 +71,41 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
     if (tid == 0) fail_col = -1;
     const int nsb = (bs + 31) >> 5;
 
-    // ---- load lower sub-tiles (padding: identity beyond bs).  Thread = row,
-    // columns in unrolled batches so LB loads per thread are in flight at once
-    // (one batch per HBM latency instead of one load per latency).
-    {
+    // ---- load lower sub-tiles (padding: identity beyond bs), with many loads
+    // per thread in flight (one batch per HBM latency).  16-byte path: lane l
+    // of wave w reads rows 4l..4l+3 of columns w, w+4, ...; it needs a 16-byte
+    // aligned block with lda and bs multiples of 4 (reads stay inside the
+    // block's columns; rows above the diagonal are read and dropped).
+    const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 4 == 0) && (bs % 4 == 0);
+    if (vec) {
+        constexpr int LB = 16;
+        const int r4 = 4 * lane, rb = r4 >> 5;
+        const int ncol = nsb * 32;
+        for (int cbase = 0; cbase < ncol; cbase += 4 * LB) {
+            float4 v[LB];
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int c = cbase + 4 * u + wave;
+                v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (c < bs && r4 < bs && r4 + 3 >= c)
+                    v[u] = *reinterpret_cast<const float4*>(A + r4 + (int64_t)c * lda);
+            }
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int c = cbase + 4 * u + wave, cb = c >> 5;
+                if (c >= ncol || rb >= nsb || rb < cb) continue;
+                const float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int r = r4 + q;
+                    float x = e[q];
+                    if (r < c) x = 0.f;
+                    if (r >= bs || c >= bs) x = (r == c) ? 1.f : 0.f;
+                    T[sidx(rb, cb) * SUBSZ + (r & 31) + (c & 31) * SUBLD] = x;
+                }
+            }
+        }
+    } else {
         constexpr int LB = 32;
         const int r = tid, rb = r >> 5;
         const int ncol = nsb * 32;
@@ -98,10 +132,17 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         float* D = T + sidx(kb, kb) * SUBSZ;
         // ---- 1. factor + invert the diagonal sub-tile (wave 0)
         if (wave == 0) {
+            // Factor and invert together, right-looking, row per lane: lane r
+            // holds row r of L (d) and of X = L^{-1} (xr).  Step k finishes
+            // column k of L and row k of X, broadcasts both through LDS (one
+            // write round, 16-byte reads), and applies the rank-1 updates.
             const int r = lane & 31;
-            float d[32];
+            float d[32], xr[32];
 #pragma unroll
-            for (int c = 0; c < 32; ++c) d[c] = (c <= r) ? D[r + c * SUBLD] : 0.f;
+            for (int c = 0; c < 32; ++c) {
+                d[c] = (c <= r) ? D[r + c * SUBLD] : 0.f;
+                xr[c] = (c == r) ? 1.f : 0.f;
+            }
             int bad = -1;
 #pragma unroll
             for (int k = 0; k < 32; ++k) {
@@ -110,42 +151,50 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
                     if (bad < 0) bad = k;
                     piv = 1.f;
                 }
-                float l = sqrtf(piv);
+                const float l = sqrtf(piv);
                 if (r == k) d[k] = l;
                 else if (r > k) d[k] = d[k] / l;
+                if (r == k) {
+                    const float rl = 1.f / l;
 #pragma unroll
-                for (int c = k + 1; c < 32; ++c) {
-                    float lck = rdlane(d[k], c);
-                    if (r >= c) d[c] = fmaf(-d[k], lck, d[c]);
+                    for (int c = 0; c <= k; ++c) xr[c] *= rl;
+                }
+                if (k < 31) {
+                    if (lane < 32) colb[k & 1][r] = d[k];
+                    if (lane == k) {
+#pragma unroll
+                        for (int c = 0; c <= k; c += 4)
+                            *reinterpret_cast<float4*>(&xb[k & 1][c]) =
+                                make_float4(xr[c], xr[c + 1], xr[c + 2], xr[c + 3]);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                    for (int c = k + 1; c < 32; ++c) {
+                        const float lck = colb[k & 1][c];
+                        if (r >= c) d[c] = fmaf(-d[k], lck, d[c]);
+                    }
+#pragma unroll
+                    for (int c = 0; c <= k; ++c) {
+                        const float xkc = xb[k & 1][c];
+                        if (r > k) xr[c] = fmaf(-d[k], xkc, xr[c]);
+                    }
                 }
             }
             // the whole sub-tile, zeros above the diagonal (d[c] = 0 for c > r):
             // the trailing updates of earlier kb wrote the full 32x32 block,
             // and the refinement below reads D as a lower-triangular operand
             if (lane < 32) {
+                float* gd = dinv + (int64_t)((col0 >> 5) + kb) * 1024;
 #pragma unroll
-                for (int c = 0; c < 32; ++c) D[r + c * SUBLD] = d[c];
+                for (int c = 0; c < 32; ++c) {
+                    D[r + c * SUBLD] = d[c];
+                    Dinv[r + c * SUBLD] = xr[c];
+                    if (kb * 32 < bs) gd[r + c * 32] = xr[c];
+                }
             }
             if (bad >= 0 && lane == 0 && kb * 32 + bad < bs && fail_col < 0)
                 fail_col = kb * 32 + bad;
-            // inverse, column per lane: X = L^{-1}, x[p] = (e_c[p] - sum L[p][q] x[q]) / L[p][p]
-            const int c = lane & 31;
-            float x[32];
-#pragma unroll
-            for (int p = 0; p < 32; ++p) {
-                float acc = (p == c) ? 1.f : 0.f;
-#pragma unroll
-                for (int q = 0; q < p; ++q) acc = fmaf(-rdlane(d[q], p), x[q], acc);
-                x[p] = acc / rdlane(d[p], p);
-            }
-            if (lane < 32) {
-                float* gd = dinv + (int64_t)((col0 >> 5) + kb) * 1024;
-#pragma unroll
-                for (int p = 0; p < 32; ++p) {
-                    Dinv[p + c * SUBLD] = x[p];
-                    if (kb * 32 < bs) gd[p + c * 32] = x[p];
-                }
-            }
         }
         __syncthreads();
 
