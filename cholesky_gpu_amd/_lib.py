@@ -57,6 +57,16 @@ class PcgInfo(C.Structure):
                 ("kappa_est", C.c_double), ("ms_total", C.c_double)]
 
 
+class GpOpts(C.Structure):
+    _fields_ = [("n_probes", C.c_int), ("probe_kind", C.c_int), ("probe_seed", C.c_uint64),
+                ("factor", FactorOpts), ("pcg", PcgOpts)]
+
+
+class GpInfo(C.Structure):
+    _fields_ = [("loss", C.c_double), ("datafit", C.c_double), ("logdet", C.c_double),
+                ("factor", FactorInfo), ("pcg", PcgInfo)]
+
+
 P = C.c_void_p
 I64 = C.c_int64
 D = C.c_double
@@ -82,6 +92,13 @@ SIGNATURES = {
                     C.POINTER(PcgInfo)]),
     "mpc_precond_cg": (I, [P, P, I64, I64, I, P, I64, P, I64, C.POINTER(FactorOpts),
                            C.POINTER(PcgOpts), C.POINTER(FactorInfo), C.POINTER(PcgInfo)]),
+    "mpc_pcg64": (I, [P, P, I64, P, I64, I64, I, I, P, I64, P, I64, C.POINTER(PcgOpts),
+                      C.POINTER(PcgInfo)]),
+    "mpc_rbf_assemble64": (I, [P, P, I64, I, I64, C.POINTER(D), I, D, D, P, I64, P, I64]),
+    "mpc_probes": (I, [P, P, I64, I64, I, C.c_uint64, I]),
+    "mpc_gp_opts_default": (None, [C.POINTER(GpOpts)]),
+    "mpc_gp_loss_grad": (I, [P, P, I64, I, I64, C.POINTER(D), I, D, D, P, C.POINTER(GpOpts), P,
+                             C.POINTER(D), C.POINTER(D), C.POINTER(D), C.POINTER(GpInfo)]),
     "mpc_quantize_panel": (I, [P, P, I64, I64, I64, I, P, P, I64, P]),
     "mpc_syrk": (I, [P, P, I64, I64, I64, P, I64, P, I, P]),
     "mpc_apply_precond": (I, [P, P, I64, I64, I, P, I64, P, I64]),

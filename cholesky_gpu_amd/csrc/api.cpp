@@ -135,7 +135,7 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->pstream) hipStreamSynchronize(h->pstream);
     for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
-                             &h->pcg, &h->scratch, &h->winv})
+                             &h->pcg, &h->scratch, &h->winv, &h->gp64, &h->gp32, &h->gpvec})
         ws_free(*w);
     if (h->ev_col) hipEventDestroy(h->ev_col);
     if (h->ev_panel) hipEventDestroy(h->ev_panel);
@@ -336,7 +336,7 @@ extern "C" int mpc_rbf_assemble(mpc_handle* h, const float* dX, int64_t n, int d
     if (rc) return rc;
     MPC_HIP(hipMemcpyAsync(h->scratch.ptr, inv.data(), sizeof(float) * n_ls, hipMemcpyHostToDevice,
                            h->stream));
-    rc = mpc_launch_rbf(h->stream, dX, n, d, ldx, (const double*)h->scratch.ptr, n_ls, variance,
+    rc = mpc_launch_rbf(h->stream, dX, n, d, ldx, (const float*)h->scratch.ptr, n_ls, variance,
                         nugget, dK, ldk);
     if (rc) return rc;
     MPC_HIP(hipStreamSynchronize(h->stream));
@@ -752,12 +752,23 @@ static int prepare_prec(mpc_handle* h, const float* dL, int64_t ldl, int64_t n) 
     return 0;
 }
 
-static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_t ldl,
-                    int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
-                    int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info) {
-    MPC_CHECK(h && dA && dL && dB && dX, "pcg: NULL argument");
+// the PCG operator: the FP32 matrix, or an FP64 one (exact products)
+struct pcg_operator {
+    const float* a32 = nullptr;
+    const double* a64 = nullptr;
+};
+static int op_apply(hipStream_t st, const pcg_operator& op, int64_t lda, int64_t n, int nrhs,
+                    const double* X, int64_t ldx, double* Y, int64_t ldy, double* part) {
+    return op.a64 ? mpc_launch_symv64(st, op.a64, lda, n, nrhs, X, ldx, Y, ldy, part)
+                  : mpc_launch_symv(st, op.a32, lda, n, nrhs, X, ldx, Y, ldy, part);
+}
+
+static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const float* dL,
+                    int64_t ldl, int64_t n, int nb, int nrhs, const double* dB, int64_t ldb,
+                    double* dX, int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info) {
+    MPC_CHECK(h && (op.a32 || op.a64) && dL && dB && dX, "pcg: NULL argument");
     MPC_CHECK(n >= 1 && lda >= n && ldl >= n && ldb >= n && ldx >= n, "pcg: bad leading dimension");
-    MPC_CHECK(nrhs >= 1 && nrhs <= 256, "pcg: nrhs must be 1..256");
+    MPC_CHECK(nrhs >= 1 && nrhs <= 4096, "pcg: nrhs must be 1..4096");
     MPC_CHECK(nb == 64 || nb == 128 || nb == 256, "pcg: nb must be 64, 128 or 256");
     mpc_pcg_opts def;
     if (!o) { mpc_pcg_opts_default(&def); o = &def; }
@@ -796,7 +807,7 @@ static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL
         if ((rc = mpc_launch_axpby(st, n, nrhs, 0.0, dX, ldx, 0.0, dX, ldx))) return rc;
     }
     // R = B - A X
-    if ((rc = mpc_launch_symv(st, dA, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
+    if ((rc = op_apply(st, op, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
     if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.R, n))) return rc;
     if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.R, n))) return rc;
     if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return rc;
@@ -828,7 +839,7 @@ static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL
             MPC_HIP(hipMemcpyAsync(b.active, act.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
             act_dirty = false;
         }
-        if ((rc = mpc_launch_symv(st, dA, lda, n, nrhs, b.P, n, b.Q, n, b.part))) return rc;
+        if ((rc = op_apply(st, op, lda, n, nrhs, b.P, n, b.Q, n, b.part))) return rc;
         if ((rc = mpc_launch_dot(st, n, nrhs, b.P, n, b.Q, n, b.pq, b.part))) return rc;
         if ((rc = mpc_launch_pcg_update_xr(st, n, nrhs, dX, ldx, b.R, n, b.P, n, b.Q, n, b.rz, b.pq,
                                            b.active)))
@@ -859,7 +870,7 @@ static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL
         for (int j = 0; j < nrhs; ++j) hrz[j] = act[j] ? hrzn[j] : hrz[j];
         if (need_true && o->stop_rule == 0) {
             // confirm on the true residual B - A X; replace R where it lags
-            if ((rc = mpc_launch_symv(st, dA, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
+            if ((rc = op_apply(st, op, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
             if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.W, n))) return rc;
             if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.W, n))) return rc;
             if ((rc = mpc_launch_dot(st, n, nrhs, b.W, n, b.W, n, b.rr, b.part))) return rc;
@@ -938,7 +949,17 @@ static int pcg_impl(mpc_handle* h, const float* dA, int64_t lda, const float* dL
 extern "C" int mpc_pcg(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_t ldl,
                        int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
                        int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info) {
-    return pcg_impl(h, dA, lda, dL, ldl, n, nb, nrhs, dB, ldb, dX, ldx, o, info);
+    pcg_operator op;
+    op.a32 = dA;
+    return pcg_impl(h, op, lda, dL, ldl, n, nb, nrhs, dB, ldb, dX, ldx, o, info);
+}
+
+extern "C" int mpc_pcg64(mpc_handle* h, const double* dA, int64_t lda, const float* dL,
+                         int64_t ldl, int64_t n, int nb, int nrhs, const double* dB, int64_t ldb,
+                         double* dX, int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info) {
+    pcg_operator op;
+    op.a64 = dA;
+    return pcg_impl(h, op, lda, dL, ldl, n, nb, nrhs, dB, ldb, dX, ldx, o, info);
 }
 
 extern "C" int mpc_precond_cg(mpc_handle* h, const float* dA, int64_t lda, int64_t n, int nrhs,
@@ -957,7 +978,155 @@ extern "C" int mpc_precond_cg(mpc_handle* h, const float* dA, int64_t lda, int64
     // the caller's A stays intact, so the fallback restores from it
     rc = factor_impl(h, L, n, n, fo, nullptr, nullptr, finfo, dA, lda);
     if (rc != 0) return rc;
-    return pcg_impl(h, dA, lda, L, n, n, fo->nb, nrhs, dB, ldb, dX, ldx, po, pinfo);
+    pcg_operator op;
+    op.a32 = dA;
+    return pcg_impl(h, op, lda, L, n, n, fo->nb, nrhs, dB, ldb, dX, ldx, po, pinfo);
+}
+
+// ---------------------------------------------------------------- GP loss and gradient
+extern "C" int mpc_rbf_assemble64(mpc_handle* h, const float* dX, int64_t n, int d, int64_t ldx,
+                                  const double* ls, int n_ls, double variance, double nugget,
+                                  double* dK64, int64_t ldk64, float* dK32, int64_t ldk32) {
+    MPC_CHECK(h && dX && ls && (dK64 || dK32), "rbf_assemble64: NULL argument");
+    MPC_CHECK(n >= 1 && d >= 1 && ldx >= d, "rbf_assemble64: bad shape");
+    MPC_CHECK(!dK64 || ldk64 >= n, "rbf_assemble64: ldk64 < n");
+    MPC_CHECK(!dK32 || ldk32 >= n, "rbf_assemble64: ldk32 < n");
+    MPC_CHECK(n_ls == 1 || n_ls == d, "rbf_assemble64: n_ls must be 1 or d");
+    for (int i = 0; i < n_ls; ++i) MPC_CHECK(ls[i] > 0, "rbf_assemble64: lengthscales must be > 0");
+    MPC_HIP(hipSetDevice(h->device));
+    std::vector<double> inv(n_ls);
+    for (int i = 0; i < n_ls; ++i) inv[i] = 1.0 / ls[i];
+    int rc = mpc_ws_reserve(h->scratch, sizeof(double) * n_ls + 256);
+    if (rc) return rc;
+    MPC_HIP(hipMemcpyAsync(h->scratch.ptr, inv.data(), sizeof(double) * n_ls, hipMemcpyHostToDevice,
+                           h->stream));
+    rc = mpc_launch_rbf64(h->stream, dX, n, d, ldx, (const double*)h->scratch.ptr, n_ls, variance,
+                          nugget, dK64, ldk64, dK32, ldk32);
+    if (rc) return rc;
+    MPC_HIP(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int mpc_probes(mpc_handle* h, double* dZ, int64_t ldz, int64_t n, int nprobe,
+                          uint64_t seed, int unit) {
+    MPC_CHECK(h && dZ, "probes: NULL argument");
+    MPC_CHECK(n >= 1 && nprobe >= 1 && ldz >= n, "probes: bad shape");
+    MPC_CHECK(nprobe <= 65535, "probes: at most 65535 probes");
+    MPC_HIP(hipSetDevice(h->device));
+    return mpc_launch_probes(h->stream, dZ, ldz, n, nprobe, seed, unit);
+}
+
+extern "C" void mpc_gp_opts_default(mpc_gp_opts* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->n_probes = 32;
+    o->probe_kind = 0;
+    o->probe_seed = 20240601ull;
+    mpc_factor_opts_default(&o->factor);
+    mpc_pcg_opts_default(&o->pcg);
+}
+
+extern "C" int mpc_gp_loss_grad(mpc_handle* h, const float* dX, int64_t n, int d, int64_t ldx,
+                                const double* ls, int n_ls, double variance, double noise,
+                                const double* dB, const mpc_gp_opts* o, double* d_grad_b,
+                                double* grad_ls, double* grad_variance, double* grad_noise,
+                                mpc_gp_info* info) {
+    MPC_CHECK(h && dX && ls && dB, "gp_loss_grad: NULL argument");
+    MPC_CHECK(n >= 1 && d >= 1 && d <= MPC_GP_MAXD && ldx >= d,
+              "gp_loss_grad: need n >= 1, 1 <= d <= %d, ldx >= d", MPC_GP_MAXD);
+    MPC_CHECK(n_ls == 1 || n_ls == d, "gp_loss_grad: n_ls must be 1 or d");
+    MPC_CHECK(variance > 0 && noise >= 0, "gp_loss_grad: variance > 0 and noise >= 0 required");
+    mpc_gp_opts def;
+    if (!o) { mpc_gp_opts_default(&def); o = &def; }
+    const int P = o->probe_kind == 1 ? (int)n : o->n_probes;
+    MPC_CHECK(P >= 0 && (o->probe_kind == 0 || o->probe_kind == 1), "gp_loss_grad: bad probe options");
+    MPC_CHECK(P + 1 <= 4096, "gp_loss_grad: at most 4095 probes");
+    MPC_HIP(hipSetDevice(h->device));
+    hipStream_t st = h->stream;
+    const int R = P + 1;
+    int rc;
+    // workspaces: A64, the FP32 copy (factor source), the factor L, vectors
+    if ((rc = mpc_ws_reserve(h->gp64, sizeof(double) * (size_t)n * n))) return rc;
+    if ((rc = mpc_ws_reserve(h->gp32, sizeof(float) * (size_t)n * n))) return rc;
+    if ((rc = mpc_ws_reserve(h->amat, sizeof(float) * (size_t)n * n))) return rc;
+    const int nout = (n_ls == 1 ? 1 : d) + 2;
+    const size_t vecR = (size_t)n * R;
+    // the partial buffer also serves the dot product b^T alpha
+    const size_t gpart = std::max<size_t>((size_t)mpc_gp_partial_doubles(n, nout),
+                                          (size_t)(n / 2048 + 72));
+    if ((rc = mpc_ws_reserve(h->gpvec, sizeof(double) * (4 * vecR + gpart + nout + 8 + n_ls))))
+        return rc;
+    double* A64 = (double*)h->gp64.ptr;
+    float* A32 = (float*)h->gp32.ptr;
+    float* L = (float*)h->amat.ptr;
+    double* RHS = (double*)h->gpvec.ptr;     // [b, z_1..z_P]
+    double* SOL = RHS + vecR;                // [alpha, A^-1 z_1, ...]
+    double* U = SOL + vecR;
+    double* V = U + vecR;
+    double* gpart_buf = V + vecR;
+    double* gout = gpart_buf + gpart;        // nout gradient sums, then logdet
+    double* dinvls = gout + nout + 8;
+
+    // 1. A in FP64 (operator) and FP32 (factor source)
+    std::vector<double> inv(n_ls);
+    for (int i = 0; i < n_ls; ++i) {
+        MPC_CHECK(ls[i] > 0, "gp_loss_grad: lengthscales must be > 0");
+        inv[i] = 1.0 / ls[i];
+    }
+    MPC_HIP(hipMemcpyAsync(dinvls, inv.data(), sizeof(double) * n_ls, hipMemcpyHostToDevice, st));
+    if ((rc = mpc_launch_rbf64(st, dX, n, d, ldx, dinvls, n_ls, variance, noise, A64, n, A32, n)))
+        return rc;
+    if ((rc = mpc_launch_copy_f32(st, n, n, A32, n, L, n))) return rc;
+    // 2. mixed-precision factor of the FP32 copy (the preconditioner)
+    mpc_factor_info fi;
+    std::memset(&fi, 0, sizeof fi);
+    rc = factor_impl(h, L, n, n, &o->factor, nullptr, nullptr, &fi, A32, n);
+    if (info) info->factor = fi;
+    if (rc != 0) return rc;
+    // 3. adjoint solve A [alpha, w_1..w_P] = [b, z_1..z_P] with the FP64 operator
+    if ((rc = mpc_launch_copy2d(st, n, 1, dB, n, RHS, n))) return rc;
+    if (P > 0 && (rc = mpc_launch_probes(st, RHS + n, n, n, P, o->probe_seed, o->probe_kind)))
+        return rc;
+    if ((rc = mpc_launch_axpby(st, n, R, 0.0, SOL, n, 0.0, SOL, n))) return rc;
+    mpc_pcg_opts po = o->pcg;
+    po.x0_mode = 0;
+    mpc_pcg_info pi;
+    std::memset(&pi, 0, sizeof pi);
+    pcg_operator op;
+    op.a64 = A64;
+    rc = pcg_impl(h, op, n, L, n, n, o->factor.nb, R, RHS, n, SOL, n, &po, &pi);
+    if (info) info->pcg = pi;
+    if (rc != 0) return rc;
+    // 4. U = [-alpha/2, w/2 A^-1 z_p], V = [alpha, z_p]
+    const double wp = o->probe_kind == 1 ? 1.0 : (P > 0 ? 1.0 / P : 0.0);
+    if ((rc = mpc_launch_axpby(st, n, 1, -0.5, SOL, n, 0.0, U, n))) return rc;
+    if (P > 0 && (rc = mpc_launch_axpby(st, n, P, 0.5 * wp, SOL + n, n, 0.0, U + n, n))) return rc;
+    if ((rc = mpc_launch_copy2d(st, n, 1, SOL, n, V, n))) return rc;
+    if (P > 0 && (rc = mpc_launch_copy2d(st, n, P, RHS + n, n, V + n, n))) return rc;
+    // 5. fused contraction against dA/dtheta, the logdet of the factor, b^T alpha
+    if ((rc = mpc_launch_gp_grad(st, dX, n, d, ldx, dinvls, n_ls, variance, U, V, n,
+                                 P > 0 ? R : 1, gpart_buf, gout)))
+        return rc;
+    if ((rc = mpc_launch_logdiag(st, L, n, n, gout + nout))) return rc;
+    if ((rc = mpc_launch_dot(st, n, 1, dB, n, SOL, n, gout + nout + 1, gpart_buf))) return rc;
+    if (d_grad_b && (rc = mpc_launch_copy2d(st, n, 1, SOL, n, d_grad_b, n))) return rc;
+    std::vector<double> hg(nout + 2);
+    MPC_HIP(hipMemcpyAsync(hg.data(), gout, sizeof(double) * (nout + 2), hipMemcpyDeviceToHost, st));
+    MPC_HIP(hipStreamSynchronize(st));
+    // d loss / d ls_c = sum m K (dx_c / ls_c)^2 / ls_c ; isotropic: sum over c
+    if (grad_ls) {
+        if (n_ls == 1) grad_ls[0] = hg[0] * inv[0];
+        else for (int c = 0; c < d; ++c) grad_ls[c] = hg[c] * inv[c];
+    }
+    const int q0 = n_ls == 1 ? 1 : d;
+    if (grad_variance) *grad_variance = hg[q0] / variance;
+    if (grad_noise) *grad_noise = hg[q0 + 1];
+    if (info) {
+        info->logdet = 2.0 * hg[nout];
+        info->datafit = 0.5 * hg[nout + 1];
+        info->loss = info->datafit + 0.5 * info->logdet;
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------- building blocks

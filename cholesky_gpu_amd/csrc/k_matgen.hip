@@ -116,36 +116,41 @@ __global__ __launch_bounds__(256) void points_kernel(float* __restrict__ X, int6
 constexpr int RT = 64;      // output tile
 constexpr int RD = 16;      // dims per LDS stage
 
+// T = float: FP32 coordinates and sums (the factor's FP32 matrix).  T =
+// double: the same kernel evaluated in FP64 (the exact operator for solves
+// that must match FP64), with an optional FP32 copy rounded from it.
+template <typename T>
 __global__ __launch_bounds__(256) void rbf_kernel(const float* __restrict__ X, int64_t n, int d,
-                                                 int64_t ldx, const float* __restrict__ inv_ls,
-                                                 int n_ls, float variance, float nugget,
-                                                 float* __restrict__ K, int64_t ldk) {
-    __shared__ float xi[RD][RT + 1];
-    __shared__ float xj[RD][RT + 1];
+                                                 int64_t ldx, const T* __restrict__ inv_ls,
+                                                 int n_ls, T variance, T nugget,
+                                                 float* __restrict__ K32, int64_t ldk32,
+                                                 double* __restrict__ K64, int64_t ldk64) {
+    __shared__ T xi[RD][RT + 1];
+    __shared__ T xj[RD][RT + 1];
     const int64_t i0 = (int64_t)blockIdx.x * RT, j0 = (int64_t)blockIdx.y * RT;
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;   // 16 x 16 threads, 4x4 each
-    float acc[4][4] = {};
+    T acc[4][4] = {};
     for (int c0 = 0; c0 < d; c0 += RD) {
         for (int idx = threadIdx.x; idx < RD * RT; idx += 256) {
             int p = idx / RD, c = idx % RD;
             int cc = c0 + c;
-            float s = cc < d ? inv_ls[n_ls == 1 ? 0 : cc] : 0.f;
+            T s = cc < d ? inv_ls[n_ls == 1 ? 0 : cc] : T(0);
             int64_t ri = i0 + p, rj = j0 + p;
-            xi[c][p] = (cc < d && ri < n) ? X[ri * ldx + cc] * s : 0.f;
-            xj[c][p] = (cc < d && rj < n) ? X[rj * ldx + cc] * s : 0.f;
+            xi[c][p] = (cc < d && ri < n) ? (T)X[ri * ldx + cc] * s : T(0);
+            xj[c][p] = (cc < d && rj < n) ? (T)X[rj * ldx + cc] * s : T(0);
         }
         __syncthreads();
 #pragma unroll
         for (int c = 0; c < RD; ++c) {
-            float a[4], b[4];
+            T a[4], b[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) { a[u] = xi[c][tx + 16 * u]; b[u] = xj[c][ty + 16 * u]; }
 #pragma unroll
             for (int u = 0; u < 4; ++u)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    float df = a[u] - b[v];
-                    acc[u][v] = fmaf(df, df, acc[u][v]);
+                    T df = a[u] - b[v];
+                    acc[u][v] = fma(df, df, acc[u][v]);
                 }
         }
         __syncthreads();
@@ -156,11 +161,29 @@ __global__ __launch_bounds__(256) void rbf_kernel(const float* __restrict__ X, i
         for (int v = 0; v < 4; ++v) {
             int64_t i = i0 + tx + 16 * u, j = j0 + ty + 16 * v;
             if (i < n && j < n) {
-                float val = variance * expf(-0.5f * acc[u][v]);
+                T val = variance * exp(T(-0.5) * acc[u][v]);
                 if (i == j) val += nugget;
-                K[i + j * ldk] = val;
+                if (K64) K64[i + j * ldk64] = (double)val;
+                if (K32) K32[i + j * ldk32] = (float)val;
             }
         }
+}
+
+// Rademacher probes z[p][i] = -1 if u < 1/2 else +1, u the Philox uniform of
+// stream 3 at index p * n + i; or unit vectors z[p] = e_p (exact trace) when
+// unit != 0.  Z is n x nprobe column-major (column p at Z + p * ldz).
+__global__ __launch_bounds__(256) void probe_kernel(double* __restrict__ Z, int64_t ldz, int64_t n,
+                                                   int nprobe, uint64_t seed, int unit) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y;
+    if (i >= n || p >= nprobe) return;
+    double v;
+    if (unit) {
+        v = (i == p) ? 1.0 : 0.0;
+    } else {
+        v = dev_uniform(seed, 3, (uint64_t)p * (uint64_t)n + (uint64_t)i) < 0.5 ? -1.0 : 1.0;
+    }
+    Z[(int64_t)p * ldz + i] = v;
 }
 
 __global__ __launch_bounds__(256) void convert_f8_kernel(const float* __restrict__ x,
@@ -233,13 +256,29 @@ int mpc_launch_points(hipStream_t st, float* X, int64_t n, int d, int64_t ldx, i
 }
 
 int mpc_launch_rbf(hipStream_t st, const float* X, int64_t n, int d, int64_t ldx,
-                   const double* dls, int n_ls, double variance, double nugget, float* K,
+                   const float* inv_ls, int n_ls, double variance, double nugget, float* K,
                    int64_t ldk) {
-    // dls: device array of inverse lengthscales (float), passed as double* for alignment
     dim3 grid((unsigned)((n + RT - 1) / RT), (unsigned)((n + RT - 1) / RT));
-    hipLaunchKernelGGL(rbf_kernel, grid, dim3(256), 0, st, X, n, d, ldx,
-                       reinterpret_cast<const float*>(dls), n_ls, (float)variance, (float)nugget, K,
-                       ldk);
+    hipLaunchKernelGGL(rbf_kernel<float>, grid, dim3(256), 0, st, X, n, d, ldx, inv_ls, n_ls,
+                       (float)variance, (float)nugget, K, ldk, (double*)nullptr, (int64_t)0);
+    MPC_LAUNCH_CHECK();
+    return 0;
+}
+
+int mpc_launch_rbf64(hipStream_t st, const float* X, int64_t n, int d, int64_t ldx,
+                     const double* inv_ls, int n_ls, double variance, double nugget, double* K64,
+                     int64_t ldk64, float* K32, int64_t ldk32) {
+    dim3 grid((unsigned)((n + RT - 1) / RT), (unsigned)((n + RT - 1) / RT));
+    hipLaunchKernelGGL(rbf_kernel<double>, grid, dim3(256), 0, st, X, n, d, ldx, inv_ls, n_ls,
+                       variance, nugget, K32, ldk32, K64, ldk64);
+    MPC_LAUNCH_CHECK();
+    return 0;
+}
+
+int mpc_launch_probes(hipStream_t st, double* Z, int64_t ldz, int64_t n, int nprobe, uint64_t seed,
+                      int unit) {
+    dim3 grid((unsigned)((n + 255) / 256), (unsigned)nprobe);
+    hipLaunchKernelGGL(probe_kernel, grid, dim3(256), 0, st, Z, ldz, n, nprobe, seed, unit);
     MPC_LAUNCH_CHECK();
     return 0;
 }

@@ -78,6 +78,9 @@ struct mpc_handle {
     mpc_workspace pcg;        // PCG vectors
     mpc_workspace scratch;    // generic
     mpc_workspace winv;       // inverses of the big diagonal blocks of L (Bo x Bo each)
+    mpc_workspace gp64;       // GP: FP64 operator A (n x n)
+    mpc_workspace gp32;       // GP: FP32 copy of A (factor source)
+    mpc_workspace gpvec;      // GP: right-hand sides, solutions, U, V, partials
     // provenance of the dinv blocks: the factor they belong to
     const float* dinv_owner = nullptr;
     int64_t      dinv_n = 0;
@@ -138,17 +141,11 @@ int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, in
 void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* class_mask,
                     unsigned* fp8_mask);
 // k_pcg.hip
-int mpc_launch_gemv(hipStream_t st, const float* A, int64_t lda, int64_t n, int nrhs,
-                    const double* X, int64_t ldx, double* Y, int64_t ldy, double* part);
 int mpc_launch_symv(hipStream_t st, const float* A, int64_t lda, int64_t n, int nrhs,
                     const double* X, int64_t ldx, double* Y, int64_t ldy, double* part);
+int mpc_launch_symv64(hipStream_t st, const double* A, int64_t lda, int64_t n, int nrhs,
+                      const double* X, int64_t ldx, double* Y, int64_t ldy, double* part);
 int64_t mpc_symv_partial_doubles(int64_t n, int nrhs);
-int mpc_launch_trsv_step_fwd(hipStream_t st, const float* L, int64_t ldl, int64_t n, int nb,
-                             int j, const float* dinv, double* W, int64_t ldw, int nrhs,
-                             double* Y, int64_t ldy);
-int mpc_launch_trsv_step_bwd(hipStream_t st, const float* L, int64_t ldl, int64_t n, int nb,
-                             int j, const float* dinv, double* W, int64_t ldw, int nrhs,
-                             double* Xo, int64_t ldxo);
 int mpc_launch_dot(hipStream_t st, int64_t n, int nrhs, const double* a, int64_t lda_,
                    const double* b, int64_t ldb, double* out, double* part);
 int mpc_launch_pcg_update_xr(hipStream_t st, int64_t n, int nrhs, double* X, int64_t ldx,
@@ -170,11 +167,21 @@ int mpc_launch_hh_matrix(hipStream_t st, int64_t n, const double* lam, const dou
 int mpc_launch_points(hipStream_t st, float* X, int64_t n, int d, int64_t ldx, int kind,
                       double p_zero, uint64_t seed);
 int mpc_launch_rbf(hipStream_t st, const float* X, int64_t n, int d, int64_t ldx,
-                   const double* dls, int n_ls, double variance, double nugget, float* K,
+                   const float* inv_ls, int n_ls, double variance, double nugget, float* K,
                    int64_t ldk);
+int mpc_launch_rbf64(hipStream_t st, const float* X, int64_t n, int d, int64_t ldx,
+                     const double* inv_ls, int n_ls, double variance, double nugget, double* K64,
+                     int64_t ldk64, float* K32, int64_t ldk32);
+constexpr int MPC_GP_MAXD = 16;   // ARD dimensions of the fused gradient contraction
+int64_t mpc_gp_partial_doubles(int64_t n, int nout);
+int mpc_launch_gp_grad(hipStream_t st, const float* X, int64_t n, int d, int64_t ldx,
+                       const double* inv_ls, int n_ls, double variance, const double* U,
+                       const double* V, int64_t ldu, int R, double* part, double* out);
+int mpc_launch_logdiag(hipStream_t st, const float* L, int64_t ldl, int64_t n, double* out);
+int mpc_launch_probes(hipStream_t st, double* Z, int64_t ldz, int64_t n, int nprobe, uint64_t seed,
+                      int unit);
 int mpc_launch_convert_f8(hipStream_t st, const float* x, uint8_t* y, int64_t count, int prec);
 int mpc_launch_dinv_from_L(hipStream_t st, const float* L, int64_t ldl, int64_t n, float* dinv);
-int mpc_gemv_nsplit(int64_t n);
 // k_tri.hip
 int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
                              const float* dinv, float* W, float* WT);

@@ -239,3 +239,101 @@ def convert_f8(h, x, prec):
     check(h.lib.mpc_convert_f8(h.h, _ptr(_dev(x)), _ptr(y), x.numel(), prec), "mpc_convert_f8")
     h.sync()
     return y
+
+
+# ---------------------------------------------------------------- GP loss / backward
+def gp_opts(n_probes=32, probe_kind=0, probe_seed=20240601, factor=None, pcg=None):
+    o = L.GpOpts()
+    L.load().mpc_gp_opts_default(C.byref(o))
+    o.n_probes, o.probe_kind, o.probe_seed = n_probes, probe_kind, probe_seed
+    if factor is not None:
+        o.factor = factor
+    if pcg is not None:
+        o.pcg = pcg
+    return o
+
+
+def gp_loss_grad(h, X, ls, variance, noise, b, opts=None):
+    """loss = b^T A^-1 b / 2 + logdet(A) / 2 for the RBF kernel matrix A of the
+    points X (n, d) float32, and its gradient.  Returns (loss, grad_b (device
+    float64), grad_ls (numpy), grad_variance, grad_noise, GpInfo)."""
+    import numpy as np
+    n, d = X.shape
+    ls = np.atleast_1d(np.asarray(ls, np.float64))
+    arr = (C.c_double * len(ls))(*ls)
+    gls = (C.c_double * len(ls))()
+    gvar, gnoise = C.c_double(0), C.c_double(0)
+    b = b.reshape(n).contiguous()
+    gb = torch.empty_like(b)
+    info = L.GpInfo()
+    opts = opts or gp_opts()
+    _pre(h)
+    check(h.lib.mpc_gp_loss_grad(h.h, _ptr(_dev(X)), n, d, d, arr, len(ls), float(variance),
+                                 float(noise), _ptr(_dev(b)), C.byref(opts), _ptr(gb), gls,
+                                 C.byref(gvar), C.byref(gnoise), C.byref(info)),
+          "mpc_gp_loss_grad")
+    return info.loss, gb, np.array(gls[:], np.float64), gvar.value, gnoise.value, info
+
+
+def probes(h, n, nprobe, seed, unit=False):
+    Z = torch.empty(nprobe, n, dtype=torch.float64, device=h.device)
+    _pre(h)
+    check(h.lib.mpc_probes(h.h, _ptr(Z), n, n, nprobe, seed, 1 if unit else 0), "mpc_probes")
+    h.sync()
+    return Z
+
+
+def rbf_assemble64(h, X, ls, variance=1.0, nugget=0.0, want32=False):
+    n, d = X.shape
+    ls = [float(v) for v in (ls if hasattr(ls, "__len__") else [ls])]
+    arr = (C.c_double * len(ls))(*ls)
+    K = torch.empty(n, n, dtype=torch.float64, device=h.device)
+    K32 = torch.empty(n, n, dtype=torch.float32, device=h.device) if want32 else None
+    _pre(h)
+    check(h.lib.mpc_rbf_assemble64(h.h, _ptr(_dev(X)), n, d, d, arr, len(ls), variance, nugget,
+                                   _ptr(K), n, _ptr(K32), n), "mpc_rbf_assemble64")
+    return (K, K32) if want32 else K
+
+
+def pcg64(h, A64, Lf, B, nb=256, opts=None, X=None):
+    """PCG with the FP64 operator A64 and the FP32 factor Lf.  B: (nrhs, n) float64."""
+    n = A64.shape[0]
+    B = B.reshape(-1, n).contiguous()
+    nrhs = B.shape[0]
+    X = X if X is not None else torch.zeros_like(B)
+    pi = L.PcgInfo()
+    opts = opts or pcg_opts()
+    _pre(h)
+    rc = h.lib.mpc_pcg64(h.h, _ptr(_dev(A64)), n, _ptr(_dev(Lf)), n, n, nb, nrhs, _ptr(_dev(B)), n,
+                         _ptr(_dev(X)), n, C.byref(opts), C.byref(pi))
+    check(rc, "mpc_pcg64", allow=(L.ERR_NOT_CONVERGED,))
+    return X, pi, rc
+
+
+class GPLoss(torch.autograd.Function):
+    """loss(ls, variance, noise, b) = b^T A^-1 b / 2 + logdet(A) / 2 for fixed
+    points X, with the hand-written backward of libmpchol (adjoint PCG +
+    fused dK/dtheta contraction, Hutchinson logdet gradient).  The forward
+    computes the gradient too; backward scales it by the incoming gradient."""
+
+    @staticmethod
+    def forward(ctx, ls, variance, noise, b, X, handle, opts):
+        loss, gb, gls, gvar, gnoise, info = gp_loss_grad(
+            handle, X, ls.detach().cpu().numpy(), float(variance), float(noise), b.detach(), opts)
+        ctx.save_for_backward(gb, torch.as_tensor(gls, dtype=ls.dtype, device=ls.device),
+                              torch.tensor(gvar, dtype=variance.dtype, device=variance.device),
+                              torch.tensor(gnoise, dtype=noise.dtype, device=noise.device))
+        ctx.info = info
+        return torch.tensor(loss, dtype=torch.float64, device=b.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        gb, gls, gvar, gnoise = ctx.saved_tensors
+        gg = g.to(torch.float64)
+        return (gg.to(gls.device) * gls, gg.to(gvar.device) * gvar, gg.to(gnoise.device) * gnoise,
+                gg * gb, None, None, None)
+
+
+def gp_loss(ls, variance, noise, b, X, handle, opts=None):
+    """Differentiable GP loss (see GPLoss); ls is a 1-element or d-element tensor."""
+    return GPLoss.apply(ls, variance, noise, b, X, handle, opts)

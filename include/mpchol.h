@@ -182,6 +182,57 @@ int mpc_precond_cg(mpc_handle* h, const float* dA, int64_t lda, int64_t n, int n
                    const mpc_factor_opts* fo, const mpc_pcg_opts* po,
                    mpc_factor_info* finfo, mpc_pcg_info* pinfo);
 
+/* PCG whose operator is an FP64 matrix: every product A p and the true
+   residual use dA (FP64, column-major); the FP32 factor dL preconditions.
+   The reference's PCG also multiplies by its FP64 A (the cublasDgemv of
+   mixed_cholesky.cu:180-217).  Used where the solution must match an FP64
+   solve (the GP gradient).  Arguments and returns as mpc_pcg. */
+int mpc_pcg64(mpc_handle* h, const double* dA, int64_t lda, const float* dL, int64_t ldl,
+              int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
+              int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info);
+
+/* ---- GP loss and hand-written backward (no reference counterpart) ------ */
+/* The RBF kernel of mpc_rbf_assemble evaluated in FP64 into dK64 (may be
+   NULL), with an FP32 copy rounded from it into dK32 (may be NULL). */
+int mpc_rbf_assemble64(mpc_handle* h, const float* dX, int64_t n, int d, int64_t ldx,
+                       const double* ls, int n_ls, double variance, double nugget,
+                       double* dK64, int64_t ldk64, float* dK32, int64_t ldk32);
+/* Trace-estimation probes, n x nprobe column-major: Rademacher z_pi = -1 if
+   the Philox uniform of stream 3 at index p*n + i is < 1/2, else +1; or the
+   unit vectors e_p when unit != 0 (nprobe <= n). */
+int mpc_probes(mpc_handle* h, double* dZ, int64_t ldz, int64_t n, int nprobe, uint64_t seed,
+               int unit);
+typedef struct {
+    int      n_probes;      /* Rademacher probes for the logdet term (default 32) */
+    int      probe_kind;    /* 0: Rademacher; 1: unit vectors, exact trace (n probes) */
+    uint64_t probe_seed;
+    mpc_factor_opts factor; /* preconditioner factor of the FP32 copy of A */
+    mpc_pcg_opts    pcg;    /* adjoint solve; x0_mode is forced to 0 */
+} mpc_gp_opts;
+typedef struct {
+    double loss;            /* datafit + logdet / 2 */
+    double datafit;         /* b^T A^-1 b / 2 (from the FP64-operator solve) */
+    double logdet;          /* 2 sum log L_ii of the preconditioner factor */
+    mpc_factor_info factor;
+    mpc_pcg_info    pcg;
+} mpc_gp_info;
+void mpc_gp_opts_default(mpc_gp_opts* o);
+/* loss = b^T A^-1 b / 2 + logdet(A) / 2 with
+   A = variance * exp(-1/2 sum_c ((x_ic - x_jc)/ls_c)^2) + noise [i==j]
+   (n_ls = 1 isotropic or d ARD, d <= 16), and its gradient:
+     d_grad_b (device, n)       = A^-1 b
+     grad_ls (host, n_ls), grad_variance, grad_noise (host; each may be NULL)
+   from one PCG solve of A [alpha, w_1..w_P] = [b, z_1..z_P] on the FP64
+   operator, preconditioned by the mixed-precision factor of the FP32 copy of
+   A, then a fused FP64 contraction of -alpha alpha^T / 2 + sum_p w_p z_p^T / (2P)
+   against dA/dtheta that recomputes kernel tiles.  The logdet gradient is the
+   Hutchinson estimate over the probes (exact for probe_kind = 1). */
+int mpc_gp_loss_grad(mpc_handle* h, const float* dX, int64_t n, int d, int64_t ldx,
+                     const double* ls, int n_ls, double variance, double noise,
+                     const double* dB, const mpc_gp_opts* o, double* d_grad_b,
+                     double* grad_ls, double* grad_variance, double* grad_noise,
+                     mpc_gp_info* info);
+
 /* ---- building blocks, exported for tests and advanced callers --------- */
 /* Quantise an m x k panel (column-major, ld) in row tiles of tile_rows rows;
    prec_of_tile is a device array of ceil(m/tile_rows) codes.  Payload is

@@ -420,6 +420,14 @@ double orc_uniform(uint64_t seed, uint64_t stream, uint64_t idx) {
     orc_philox4x32(ctr, key, o);
     return u53(o[0], o[1]);
 }
+/* trace-estimation probes (no reference counterpart): Z n x nprobe column-major,
+   z_pi = -1 if uniform(seed, 3, p*n + i) < 1/2 else +1 */
+void orc_probes(int64_t n, int nprobe, uint64_t seed, double* Z, int64_t ldz) {
+    for (int p = 0; p < nprobe; ++p)
+        for (int64_t i = 0; i < n; ++i)
+            Z[(int64_t)p * ldz + i] = orc_uniform(seed, 3, (uint64_t)p * (uint64_t)n + (uint64_t)i) < 0.5
+                                          ? -1.0 : 1.0;
+}
 double orc_normal(uint64_t seed, uint64_t stream, uint64_t idx) {
     uint64_t q = idx >> 1;
     uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)stream, 0x4e4f524du};

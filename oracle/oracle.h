@@ -83,6 +83,8 @@ int orc_quantize_panel(const float* P, int64_t m, int64_t k, int64_t ld,
 void   orc_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* element idx of stream `stream` under `seed`: uniform in [0,1) with 53 bits */
 double orc_uniform(uint64_t seed, uint64_t stream, uint64_t idx);
+/* Rademacher trace probes: z_pi = -1 if orc_uniform(seed, 3, p*n + i) < 1/2, else +1 */
+void orc_probes(int64_t n, int nprobe, uint64_t seed, double* Z, int64_t ldz);
 /* standard normal via Box-Muller on the pair (2*(idx/2), +1) */
 double orc_normal(uint64_t seed, uint64_t stream, uint64_t idx);
 

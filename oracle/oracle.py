@@ -91,6 +91,7 @@ def _declare(L):
         "orc_quantize_panel": (C.c_int, [P, I64, I64, I64, C.c_int, P, P, I64, P, P, I64]),
         "orc_philox4x32": (None, [P, P, P]),
         "orc_uniform": (D, [U64, U64, U64]),
+        "orc_probes": (None, [I64, C.c_int, U64, P, I64]),
         "orc_normal": (D, [U64, U64, U64]),
         "orc_spectrum": (None, [C.c_int, I64, D, D, I64, D, D, U64, P]),
         "orc_hh_setup": (C.c_int, [I64, C.c_int, U64, P, P, P, P]),
@@ -207,6 +208,69 @@ def normal(seed, stream, idx):
 
 def uniform(seed, stream, idx):
     return lib().orc_uniform(seed, stream, idx)
+
+
+def probes(n, nprobe, seed):
+    """(nprobe, n) float64 Rademacher probes (orc_probes)."""
+    Z = np.zeros((nprobe, n), np.float64)
+    lib().orc_probes(n, nprobe, seed, _p(Z), n)
+    return Z
+
+
+def _gp_sqdiff(Xs, c):
+    """(x_ic - x_jc)^2 of the scaled points for dimension c (n, n)."""
+    x = Xs[:, c]
+    return (x[:, None] - x[None, :]) ** 2
+
+
+def gp_matrix_f64(X, ls, variance, noise):
+    """A = variance * exp(-1/2 sum_c ((x_ic - x_jc)/ls_c)^2) + noise I in FP64
+    (X: (n, d) float32), the noise-free kernel K0, and the scaled points."""
+    X = np.asarray(X, np.float64)
+    ls = np.atleast_1d(np.asarray(ls, np.float64))
+    Xs = X / ls if ls.size > 1 else X / ls[0]
+    S = np.zeros((X.shape[0], X.shape[0]))
+    for c in range(X.shape[1]):
+        S += _gp_sqdiff(Xs, c)
+    K0 = variance * np.exp(-0.5 * S)
+    return K0 + noise * np.eye(X.shape[0]), K0, Xs
+
+
+def gp_loss_grad_f64(X, ls, variance, noise, b, Z=None, unit=False):
+    """FP64 reference of the GP loss b^T A^-1 b / 2 + logdet(A) / 2 and its
+    gradient.  With Z (nprobe, n) the logdet gradient is the Hutchinson estimate
+    over those probes (weight 1/P, or 1 with unit=True); with Z None it is exact
+    (tr(A^-1 dA)).  Returns loss, grad_b, grad_ls, grad_variance, grad_noise.
+    dA_ij/dls_c = K_ij (x_ic - x_jc)^2 / ls_c^3, dA/dvariance = K / variance,
+    dA/dnoise = I.  (No reference counterpart: the GP workload is defined by
+    the differentiable-solve configuration only.)"""
+    import scipy.linalg as sl
+    ls = np.atleast_1d(np.asarray(ls, np.float64))
+    A, K0, Xs = gp_matrix_f64(X, ls, variance, noise)
+    cf = sl.cho_factor(A, lower=True)
+    del A
+    b = np.asarray(b, np.float64)
+    alpha = sl.cho_solve(cf, b)
+    logdet = 2.0 * np.log(np.diag(cf[0])).sum()
+    loss = 0.5 * b @ alpha + 0.5 * logdet
+    if Z is None:
+        T = sl.cho_solve(cf, np.eye(K0.shape[0]))           # exact: tr(A^-1 dA)
+    else:
+        W = sl.cho_solve(cf, Z.T)                           # A^-1 z_p as columns
+        T = W @ Z / (1.0 if unit else Z.shape[0])
+    del cf
+    M = 0.5 * T
+    del T
+    M -= 0.5 * np.outer(alpha, alpha)
+    gnoise = np.trace(M)
+    MK = M * K0                                             # weights of (dx/ls)^2 per dim
+    del M, K0
+    d = Xs.shape[1]
+    if ls.size > 1:
+        gls = np.array([(MK * _gp_sqdiff(Xs, c)).sum() / ls[c] for c in range(d)])
+    else:
+        gls = np.array([sum((MK * _gp_sqdiff(Xs, c)).sum() for c in range(d)) / ls[0]])
+    return loss, alpha, gls, MK.sum() / variance, gnoise
 
 
 def spectrum(dist, n, cond, lam_max=1.0, n_cluster=0, cluster_lo=0.0, cluster_hi=0.0, seed=0):

@@ -291,3 +291,62 @@ def test_factor_breakdown_info():
     A[100, 100] = -1
     L, rc, _, _, _ = O.factor_mixed(A, nb=64, update_prec=O.FP32)
     assert rc == 101
+
+
+# ---------------------------------------------------------------- GP loss and gradient
+def _gp_case(n=96, d=3, seed=0):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal((n, d)).astype(np.float32)
+    return X, np.array([0.7, 1.3, 1.9][:d]), 1.3, 0.05, rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("ard", [True, False])
+def test_gp_gradient_matches_finite_differences(ard):
+    """The exact FP64 gradient formula (dA/dls = K dx^2/ls^3, dA/dvar = K/var,
+    dA/dnoise = I, dloss/db = A^-1 b) against central differences of the
+    exact loss: a dropped term, a wrong power of ls or a missing 1/2 fails."""
+    X, ls, var, noise, b = _gp_case()
+    if not ard:
+        ls = np.array([1.1])
+    loss, gb, gls, gvar, gno = O.gp_loss_grad_f64(X, ls, var, noise, b)
+
+    def f(ls_, var_, no_, b_):
+        return O.gp_loss_grad_f64(X, ls_, var_, no_, b_)[0]
+
+    h = 1e-5
+    for c in range(ls.size):
+        e = np.zeros(ls.size)
+        e[c] = h
+        fd = (f(ls + e, var, noise, b) - f(ls - e, var, noise, b)) / (2 * h)
+        assert abs(fd - gls[c]) <= 1e-6 * max(1.0, abs(fd)), (c, fd, gls[c])
+    fd = (f(ls, var + h, noise, b) - f(ls, var - h, noise, b)) / (2 * h)
+    assert abs(fd - gvar) <= 1e-6 * max(1.0, abs(fd))
+    hn = 1e-7
+    fd = (f(ls, var, noise + hn, b) - f(ls, var, noise - hn, b)) / (2 * hn)
+    assert abs(fd - gno) <= 1e-5 * max(1.0, abs(fd))
+    for i in (0, 17, 95):
+        e = np.zeros(b.size)
+        e[i] = h
+        fd = (f(ls, var, noise, b + e) - f(ls, var, noise, b - e)) / (2 * h)
+        assert abs(fd - gb[i]) <= 1e-6 * max(1.0, abs(fd))
+
+
+def test_gp_unit_probes_give_exact_trace():
+    """Hutchinson with the unit vectors e_1..e_n (weight 1) is the exact trace."""
+    X, ls, var, noise, b = _gp_case(n=64)
+    ex = O.gp_loss_grad_f64(X, ls, var, noise, b)
+    un = O.gp_loss_grad_f64(X, ls, var, noise, b, Z=np.eye(64), unit=True)
+    for a, c in zip(ex[2:], un[2:]):
+        assert np.allclose(a, c, rtol=1e-12, atol=1e-12)
+
+
+def test_probes_are_rademacher_and_seeded():
+    Z = O.probes(4096, 8, 99)
+    assert set(np.unique(Z)) == {-1.0, 1.0}
+    assert abs(Z.mean()) < 0.02                     # balanced
+    assert np.array_equal(Z, O.probes(4096, 8, 99))
+    assert not np.array_equal(Z, O.probes(4096, 8, 100))
+    # definition: sign of the Philox uniform of stream 3 at index p*n + i
+    for p, i in [(0, 0), (3, 17), (7, 4095)]:
+        u = O.uniform(99, 3, p * 4096 + i)
+        assert Z[p, i] == (-1.0 if u < 0.5 else 1.0)
