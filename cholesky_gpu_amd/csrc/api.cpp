@@ -719,7 +719,7 @@ int apply_prec(mpc_handle* h, const float* L, int64_t ldl, int64_t n, int nb, in
     const int64_t nI = (n + h->winv_bo - 1) / h->winv_bo;
     const float* Wv = (const float*)h->winv.ptr;
     return mpc_apply_block_prec(st, L, ldl, n, h->winv_bo, Wv, Wv + (size_t)nI * h->winv_bo * h->winv_bo,
-                                b.W, b.Y, Z, n, nrhs);
+                                b.W, b.Y, Z, n, nrhs, b.part);
 }
 }  // namespace
 
@@ -778,8 +778,10 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     int rc;
     if ((rc = prepare_prec(h, dL, ldl, n))) return rc;
     const size_t vec = (size_t)n * nrhs;
-    const size_t part = std::max<size_t>((size_t)mpc_symv_partial_doubles(n, nrhs),
-                                         (size_t)nrhs * ((n + 2047) / 2048) + 64);
+    const size_t part = std::max<size_t>(
+        std::max<size_t>((size_t)mpc_symv_partial_doubles(n, nrhs),
+                         (size_t)mpc_block_prec_partial_doubles(mpc_block_bo(n), nrhs)),
+        (size_t)nrhs * ((n + 2047) / 2048) + 64);
     const size_t nbytes = sizeof(double) * (6 * vec + part + 5 * (size_t)nrhs + 64) + sizeof(int) * nrhs + 1024;
     if ((rc = mpc_ws_reserve(h->pcg, nbytes))) return rc;
     pcg_bufs b;
@@ -1177,16 +1179,18 @@ extern "C" int mpc_apply_precond(mpc_handle* h, const float* dL, int64_t ldl, in
     int rc = prepare_prec(h, dL, ldl, n);
     if (rc) return rc;
     const size_t vec = (size_t)n * nrhs;
-    if ((rc = mpc_ws_reserve(h->scratch, sizeof(double) * 3 * vec))) return rc;
+    const size_t part = (size_t)mpc_block_prec_partial_doubles(h->winv_bo, nrhs);
+    if ((rc = mpc_ws_reserve(h->scratch, sizeof(double) * (3 * vec + part)))) return rc;
     double* W = (double*)h->scratch.ptr;
     double* Y = W + vec;
     double* Z = Y + vec;
+    double* P = Z + vec;
     hipStream_t st = h->stream;
     if ((rc = mpc_launch_copy2d(st, n, nrhs, dR, ldr, W, n))) return rc;
     const int64_t nI = (n + h->winv_bo - 1) / h->winv_bo;
     const float* Wv = (const float*)h->winv.ptr;
     if ((rc = mpc_apply_block_prec(st, dL, ldl, n, h->winv_bo, Wv,
-                                   Wv + (size_t)nI * h->winv_bo * h->winv_bo, W, Y, Z, n, nrhs)))
+                                   Wv + (size_t)nI * h->winv_bo * h->winv_bo, W, Y, Z, n, nrhs, P)))
         return rc;
     if ((rc = mpc_launch_copy2d(st, n, nrhs, Z, n, dZ, ldz))) return rc;
     MPC_HIP(hipStreamSynchronize(st));

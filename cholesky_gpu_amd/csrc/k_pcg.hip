@@ -27,6 +27,8 @@ constexpr int NRC = 8;           // right-hand sides per pass
 // matrix element type: 128 for FP32 (65 KB of LDS), 64 for FP64 (33 KB).
 template <typename T> struct symv_tile { static constexpr int ST = 128; };
 template <> struct symv_tile<double> { static constexpr int ST = 64; };
+template <typename T> struct vec16 { using type = float4; };
+template <> struct vec16<double> { using type = double2; };
 
 template <typename T>
 __global__ __launch_bounds__(256) void symv_tile_kernel(const T* __restrict__ A, int64_t lda,
@@ -46,12 +48,39 @@ __global__ __launch_bounds__(256) void symv_tile_kernel(const T* __restrict__ A,
     const int tid = threadIdx.x;
     const int64_t r0 = I * ST, c0 = J * ST;
     const bool diag = I == J;
-    for (int idx = tid; idx < ST * ST; idx += 256) {
-        const int r = idx % ST, c = idx / ST;
-        const int64_t gr = r0 + r, gc = c0 + c;
-        T v = 0;
-        if (gr < n && gc < n && (!diag || r <= c)) v = A[gr + gc * lda];
-        As[c * (ST + 1) + r] = v;
+    // Stage the tile.  Full tiles of an aligned matrix: 16-byte loads, all of
+    // a thread's loads issued before any LDS store (one HBM latency per tile).
+    using V16 = typename vec16<T>::type;
+    constexpr int VE = 16 / (int)sizeof(T);        // elements per 16-byte load
+    constexpr int RV = ST / VE;                     // loads per tile column
+    constexpr int CPP = 256 / RV;                   // columns per pass
+    constexpr int NP = ST / CPP;                    // passes
+    const bool fast = r0 + ST <= n && c0 + ST <= n && lda % VE == 0 &&
+                      (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+    if (fast) {
+        const int q = tid % RV, cc = tid / RV;
+        V16 v[NP];
+#pragma unroll
+        for (int p = 0; p < NP; ++p)
+            v[p] = *reinterpret_cast<const V16*>(A + r0 + VE * q + (c0 + p * CPP + cc) * lda);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+            const int c = p * CPP + cc;
+            const T* e = reinterpret_cast<const T*>(&v[p]);
+#pragma unroll
+            for (int u = 0; u < VE; ++u) {
+                const int r = VE * q + u;
+                As[c * (ST + 1) + r] = (!diag || r <= c) ? e[u] : T(0);
+            }
+        }
+    } else {
+        for (int idx = tid; idx < ST * ST; idx += 256) {
+            const int r = idx % ST, c = idx / ST;
+            const int64_t gr = r0 + r, gc = c0 + c;
+            T v = 0;
+            if (gr < n && gc < n && (!diag || r <= c)) v = A[gr + gc * lda];
+            As[c * (ST + 1) + r] = v;
+        }
     }
     for (int j = 0; j < nrhs; ++j) {
         if (tid < ST) {

@@ -124,30 +124,42 @@ struct tile_ctx {
 // BLAS GEMM with beta = 1 does: seeding the accumulator with -C instead would
 // round every one of the k FMA steps at the scale of |C| rather than of the
 // partial sum, about sqrt(k) times the error for a trailing update.
+// Addresses: element (a, b, i) = wave base + column offset (32b + 8(i/4) +
+// i%4) * ldc, which is wave-uniform (scalar registers), + the lane's byte
+// offset (32a + n + 4h ldc) * 4, one 32-bit VGPR per a.
 __device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[2][2]) {
     const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
-    float* base = t.C + t.rowbase + t.colbase * t.ldc;
-    const int ld = (int)t.ldc;
+    const char* cbase = reinterpret_cast<const char*>(t.C + t.rowbase + t.colbase * t.ldc);
+    char* wbase = reinterpret_cast<char*>(t.C + t.rowbase + t.colbase * t.ldc);
+    const int64_t ld = t.ldc;
     const int rlim = (int)(t.m - t.rowbase), clim = (int)(t.m - t.colbase);
+    uint32_t loff[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) loff[a] = (uint32_t)((32 * a + n) + 4 * h * ld) * 4u;
+    // all 64 loads first (one HBM latency), then subtract and store
+    float v[2][2][16];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int r = 32 * a + n;
-            float v[16];
+        for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const int c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
+                const int r = 32 * a + n, c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
                 const bool ok = r < rlim && c < clim && (!t.diag || r >= c);
-                v[i] = ok ? base[r + c * ld] : 0.f;
+                const char* col = cbase + (int64_t)(32 * b + 8 * (i >> 2) + (i & 3)) * ld * 4;
+                v[a][b][i] = ok ? *reinterpret_cast<const float*>(col + loff[a]) : 0.f;
             }
 #pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const int c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
+                const int r = 32 * a + n, c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
                 const bool ok = r < rlim && c < clim && (!t.diag || r >= c);
-                if (ok) base[r + c * ld] = v[i] - acc[a][b][i];
+                char* col = wbase + (int64_t)(32 * b + 8 * (i >> 2) + (i & 3)) * ld * 4;
+                if (ok) *reinterpret_cast<float*>(col + loff[a]) = v[a][b][i] - acc[a][b][i];
             }
-        }
 }
 
 // FP8 class: A operand = column-side rows, B operand = row-side rows.

@@ -141,22 +141,21 @@ __global__ __launch_bounds__(256) void transpose_blocks_kernel(const float* __re
 
 // ---------------------------------------------------------------- TRSV kernels
 constexpr int NR = 4;   // right-hand sides per pass
-constexpr int UR = 16;  // loads in flight per lane
 
 // W[r][j] -= sum_{c < cw} L[r][c0 + c] * Y[c0 + c][j]  for r in [r0, n)
-// 64 rows per workgroup (lane = row, 256 B per column load), 8 waves split
-// the columns into contiguous ranges, UR independent loads per batch.
-__global__ __launch_bounds__(512) void fwd_update_kernel(const float* __restrict__ L, int64_t ldl,
-                                                        int64_t n, int64_t r0, int64_t c0, int cw,
-                                                        const double* __restrict__ Y, int64_t ldy,
-                                                        double* __restrict__ W, int64_t ldw,
-                                                        int nrhs) {
-    __shared__ double red[8][NR][64];
+// 64 rows per workgroup (lane = row, 256 B per column load); 16 waves split
+// the columns into contiguous ranges; 32 independent loads per batch.
+constexpr int FU_WAVES = 16;
+constexpr int FU_UR = 32;
+__global__ __launch_bounds__(64 * FU_WAVES) void fwd_update_kernel(
+    const float* __restrict__ L, int64_t ldl, int64_t n, int64_t r0, int64_t c0, int cw,
+    const double* __restrict__ Y, int64_t ldy, double* __restrict__ W, int64_t ldw, int nrhs) {
+    __shared__ double red[FU_WAVES][NR][64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int64_t r = r0 + (int64_t)blockIdx.x * 64 + lane;
     const int j0 = blockIdx.y * NR;
     const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
-    const int per = (cw + 7) / 8;
+    const int per = (cw + FU_WAVES - 1) / FU_WAVES;
     const int cb = wave * per, ce = cb + per < cw ? cb + per : cw;
     double acc[NR];
 #pragma unroll
@@ -165,16 +164,16 @@ __global__ __launch_bounds__(512) void fwd_update_kernel(const float* __restrict
     const float* Lr = L + rr + c0 * ldl;
     const double* Yc = Y + c0 + (int64_t)j0 * ldy;
     int c = cb;
-    for (; c + UR <= ce; c += UR) {
-        float l[UR];
+    for (; c + FU_UR <= ce; c += FU_UR) {
+        float l[FU_UR];
 #pragma unroll
-        for (int u = 0; u < UR; ++u) l[u] = Lr[(int64_t)(c + u) * ldl];
+        for (int u = 0; u < FU_UR; ++u) l[u] = Lr[(int64_t)(c + u) * ldl];
 #pragma unroll
         for (int j = 0; j < NR; ++j)
             if (j < nj) {
                 const double* y = Yc + (int64_t)j * ldy + c;
 #pragma unroll
-                for (int u = 0; u < UR; ++u) acc[j] = fma((double)l[u], y[u], acc[j]);
+                for (int u = 0; u < FU_UR; ++u) acc[j] = fma((double)l[u], y[u], acc[j]);
             }
     }
     for (; c < ce; ++c) {
@@ -192,112 +191,103 @@ __global__ __launch_bounds__(512) void fwd_update_kernel(const float* __restrict
             if (j < nj) {
                 double s = 0.0;
 #pragma unroll
-                for (int w = 0; w < 8; ++w) s += red[w][j][lane];
+                for (int w = 0; w < FU_WAVES; ++w) s += red[w][j][lane];
                 W[r + (int64_t)(j0 + j) * ldw] -= s;
             }
     }
 }
 
-// wave-per-vector dot of a contiguous float segment with FP64 vectors:
-// out[j] = sum_{t < len} seg[t] * V[off + t + j*ldv]
-__device__ __forceinline__ void seg_dot(const float* __restrict__ seg, int len,
-                                        const double* __restrict__ V, int64_t off, int64_t ldv,
-                                        int nj, double out[NR]) {
-    const int lane = threadIdx.x & 63;
-    double acc[NR];
+// Triangular GEMV on a diagonal big block, out[i] = sum_j M[i + j*ld] v[j]
+// over j <= i (lower: M = Winv, forward) or j >= i (upper: M = Winv^T,
+// backward), i, j < bI.  One wave per 64 x 64 tile (row block ib, column
+// chunk jb) of the triangle; lane = row; the tile's 64 column loads are all in
+// flight at once.  Partial sums go to part[(jb * nrhs + j) * bI + i];
+// tri_gemv_reduce adds the chunks of a row in order (deterministic).
+constexpr int TG = 64;
+__global__ __launch_bounds__(64) void tri_gemv_tile_kernel(const float* __restrict__ M, int ld,
+                                                          int bI, int upper,
+                                                          const double* __restrict__ V,
+                                                          int64_t ldv, int nrhs,
+                                                          double* __restrict__ part) {
+    const int nb = (bI + TG - 1) / TG;
+    // tile t -> (ib, jb): lower ib >= jb, t = ib (ib + 1) / 2 + jb; upper mirrors it
+    const int t = blockIdx.x;
+    int a = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (a * (a + 1) / 2 > t) --a;
+    while ((a + 1) * (a + 2) / 2 <= t) ++a;
+    const int b = t - a * (a + 1) / 2;
+    const int ib = upper ? b : a, jb = upper ? a : b;
+    (void)nb;
+    const int lane = threadIdx.x;
+    const int i = ib * TG + lane;
+    const int ic = i < bI ? i : bI - 1;             // clamp: loads stay in bounds
+    const int j0 = jb * TG;
+    const int jn = bI - j0 < TG ? bI - j0 : TG;
+    const int r0 = blockIdx.y * NR;
+    const int nj = nrhs - r0 < NR ? nrhs - r0 : NR;
+    float m[TG];
 #pragma unroll
-    for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-    int t = lane;
-    for (; t + 64 * (UR / 2 - 1) < len; t += 64 * (UR / 2)) {
-        float s[UR / 2];
-#pragma unroll
-        for (int u = 0; u < UR / 2; ++u) s[u] = seg[t + 64 * u];
-#pragma unroll
-        for (int j = 0; j < NR; ++j)
-            if (j < nj) {
-                const double* v = V + off + (int64_t)j * ldv + t;
-#pragma unroll
-                for (int u = 0; u < UR / 2; ++u) acc[j] = fma((double)s[u], v[64 * u], acc[j]);
-            }
+    for (int u = 0; u < TG; ++u) {
+        const int j = j0 + u;
+        const bool keep = u < jn && (upper ? j >= ic : j <= ic);
+        m[u] = keep ? M[ic + (int64_t)j * ld] : 0.f;
     }
-    for (; t < len; t += 64) {
-        float s0 = seg[t];
+    for (int r = 0; r < nj; ++r) {
+        const double* v = V + (int64_t)(r0 + r) * ldv + j0;
+        double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < NR; ++j)
-            if (j < nj) acc[j] = fma((double)s0, V[off + (int64_t)j * ldv + t], acc[j]);
-    }
-#pragma unroll
-    for (int j = 0; j < NR; ++j) {
-        double s = acc[j];
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o);
-        out[j] = s;
-    }
-}
-
-// Y[p0 + r][j] = sum_{c <= r} Winv[r][c] W[p0 + c][j]: row r of Winv is column
-// r of WT (contiguous).  One wave per row, 4 rows per wave.
-__global__ __launch_bounds__(256) void fwd_diag_kernel(const float* __restrict__ WTi, int Bo, int bI,
-                                                      int64_t p0, const double* __restrict__ W,
-                                                      int64_t ldw, double* __restrict__ Y,
-                                                      int64_t ldy, int nrhs) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j0 = blockIdx.y * NR;
-    const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
-    for (int q = 0; q < 4; ++q) {
-        const int r = blockIdx.x * 16 + wave * 4 + q;
-        if (r >= bI) break;
-        double out[NR];
-        seg_dot(WTi + (int64_t)r * Bo, r + 1, W, p0 + (int64_t)j0 * ldw, ldw, nj, out);
-        if (lane == 0) {
-#pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (j < nj) Y[p0 + r + (int64_t)(j0 + j) * ldy] = out[j];
-        }
+        for (int u = 0; u < TG; ++u)
+            if (u < jn) acc = fma((double)m[u], v[u], acc);
+        if (i < bI) part[((int64_t)jb * nrhs + r0 + r) * bI + i] = acc;
     }
 }
 
-// X[p0 + c][j] = sum_{r >= c} Winv[r][c] W[p0 + r][j]: column c of Winv
-__global__ __launch_bounds__(256) void bwd_diag_kernel(const float* __restrict__ Wi, int Bo, int bI,
-                                                      int64_t p0, const double* __restrict__ W,
-                                                      int64_t ldw, double* __restrict__ X,
-                                                      int64_t ldx, int nrhs) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int j0 = blockIdx.y * NR;
-    const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
-    for (int q = 0; q < 4; ++q) {
-        const int c = blockIdx.x * 16 + wave * 4 + q;
-        if (c >= bI) break;
-        double out[NR];
-        seg_dot(Wi + c + (int64_t)c * Bo, bI - c, W, p0 + c + (int64_t)j0 * ldw, ldw, nj, out);
-        if (lane == 0) {
-#pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (j < nj) X[p0 + c + (int64_t)(j0 + j) * ldx] = out[j];
-        }
-    }
+__global__ __launch_bounds__(256) void tri_gemv_reduce_kernel(const double* __restrict__ part,
+                                                             int bI, int nrhs, int upper,
+                                                             double* __restrict__ out,
+                                                             int64_t ldo) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int r = blockIdx.y;
+    if (i >= bI) return;
+    const int nb = (bI + TG - 1) / TG, ib = i / TG;
+    const int jlo = upper ? ib : 0, jhi = upper ? nb - 1 : ib;
+    double s = 0.0;
+    for (int jb = jlo; jb <= jhi; ++jb) s += part[((int64_t)jb * nrhs + r) * bI + i];
+    out[i + (int64_t)r * ldo] = s;
 }
 
 // W[c][j] -= sum_{r < bI} L[p0 + r][c] * X[p0 + r][j]   for c < p0
-// one wave per column (contiguous 4*bI bytes), 4 columns per wave
+// one wave per column c (a contiguous run of bI floats): lane r holds rows
+// r, r + 64, ... (all loads in flight), then a wave reduction per rhs.
+constexpr int BU_MAXL = 32;     // bI <= 64 * BU_MAXL = 2048
 __global__ __launch_bounds__(256) void bwd_update_kernel(const float* __restrict__ L, int64_t ldl,
                                                         int64_t p0, int bI,
                                                         const double* __restrict__ X, int64_t ldx,
                                                         double* __restrict__ W, int64_t ldw,
                                                         int nrhs) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t c = (int64_t)blockIdx.x * 4 + wave;
+    if (c >= p0) return;
     const int j0 = blockIdx.y * NR;
     const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
-    for (int q = 0; q < 4; ++q) {
-        const int64_t c = (int64_t)blockIdx.x * 16 + wave * 4 + q;
-        if (c >= p0) break;
-        double out[NR];
-        seg_dot(L + p0 + c * ldl, bI, X, p0 + (int64_t)j0 * ldx, ldx, nj, out);
-        if (lane == 0) {
+    const float* col = L + p0 + c * ldl;
+    float l[BU_MAXL];
 #pragma unroll
-            for (int j = 0; j < NR; ++j)
-                if (j < nj) W[c + (int64_t)(j0 + j) * ldw] -= out[j];
+    for (int u = 0; u < BU_MAXL; ++u) {
+        const int rr = lane + 64 * u;
+        l[u] = rr < bI ? col[rr] : 0.f;
+    }
+    for (int j = 0; j < nj; ++j) {
+        const double* x = X + p0 + (int64_t)(j0 + j) * ldx;
+        double acc = 0.0;
+#pragma unroll
+        for (int u = 0; u < BU_MAXL; ++u) {
+            const int rr = lane + 64 * u;
+            if (rr < bI) acc = fma((double)l[u], x[rr], acc);
         }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) W[c + (int64_t)(j0 + j) * ldw] -= acc;
     }
 }
 
@@ -389,30 +379,48 @@ int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_
 }
 
 // z = L^{-T} L^{-1} w for nrhs columns.  W (in/out, destroyed), Y (work), Z (out)
+static int tri_gemv(hipStream_t st, const float* M, int Bo, int bI, int upper, const double* V,
+                    int64_t ldv, int nrhs, double* part, double* out, int64_t ldo) {
+    const int nb = (bI + TG - 1) / TG;
+    hipLaunchKernelGGL(tri_gemv_tile_kernel, dim3((unsigned)(nb * (nb + 1) / 2), cdiv(nrhs, NR)),
+                       dim3(64), 0, st, M, Bo, bI, upper, V, ldv, nrhs, part);
+    hipLaunchKernelGGL(tri_gemv_reduce_kernel, dim3(cdiv(bI, 256), (unsigned)nrhs), dim3(256), 0, st,
+                       part, bI, nrhs, upper, out, ldo);
+    MPC_LAUNCH_CHECK();
+    return 0;
+}
+
+int64_t mpc_block_prec_partial_doubles(int Bo, int nrhs) {
+    return (int64_t)((Bo + TG - 1) / TG) * Bo * nrhs;
+}
+
 int mpc_apply_block_prec(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
                          const float* Winv, const float* WinvT, double* W, double* Y, double* Z,
-                         int64_t ld, int nrhs) {
+                         int64_t ld, int nrhs, double* part) {
+    if (Bo > 64 * BU_MAXL) {
+        mpc_set_error("apply_block_prec: big block %d exceeds %d", Bo, 64 * BU_MAXL);
+        return MPC_ERR_ARG;
+    }
     const int64_t nI = (n + Bo - 1) / Bo;
     const unsigned gz = cdiv(nrhs, NR);
+    int rc;
     for (int64_t I = 0; I < nI; ++I) {
         const int64_t p0 = I * Bo;
         const int bI = (int)(n - p0 < Bo ? n - p0 : Bo);
-        const float* WTi = WinvT + I * (int64_t)Bo * Bo;
-        hipLaunchKernelGGL(fwd_diag_kernel, dim3(cdiv(bI, 16), gz), dim3(256), 0, st, WTi, Bo, bI,
-                           p0, W, ld, Y, ld, nrhs);
+        const float* Wi = Winv + I * (int64_t)Bo * Bo;
+        if ((rc = tri_gemv(st, Wi, Bo, bI, 0, W + p0, ld, nrhs, part, Y + p0, ld))) return rc;
         const int64_t r0 = p0 + bI;
         if (r0 < n)
-            hipLaunchKernelGGL(fwd_update_kernel, dim3(cdiv(n - r0, 64), gz), dim3(512), 0, st, L,
-                               ldl, n, r0, p0, bI, Y, ld, W, ld, nrhs);
+            hipLaunchKernelGGL(fwd_update_kernel, dim3(cdiv(n - r0, 64), gz), dim3(64 * FU_WAVES), 0,
+                               st, L, ldl, n, r0, p0, bI, Y, ld, W, ld, nrhs);
     }
     for (int64_t I = nI - 1; I >= 0; --I) {
         const int64_t p0 = I * Bo;
         const int bI = (int)(n - p0 < Bo ? n - p0 : Bo);
-        const float* Wi = Winv + I * (int64_t)Bo * Bo;
-        hipLaunchKernelGGL(bwd_diag_kernel, dim3(cdiv(bI, 16), gz), dim3(256), 0, st, Wi, Bo, bI,
-                           p0, Y, ld, Z, ld, nrhs);
+        const float* WTi = WinvT + I * (int64_t)Bo * Bo;
+        if ((rc = tri_gemv(st, WTi, Bo, bI, 1, Y + p0, ld, nrhs, part, Z + p0, ld))) return rc;
         if (p0 > 0)
-            hipLaunchKernelGGL(bwd_update_kernel, dim3(cdiv(p0, 16), gz), dim3(256), 0, st, L, ldl,
+            hipLaunchKernelGGL(bwd_update_kernel, dim3(cdiv(p0, 4), gz), dim3(256), 0, st, L, ldl,
                                p0, bI, Z, ld, Y, ld, nrhs);
     }
     MPC_LAUNCH_CHECK();

@@ -185,9 +185,11 @@ int mpc_launch_dinv_from_L(hipStream_t st, const float* L, int64_t ldl, int64_t 
 // k_tri.hip
 int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
                              const float* dinv, float* W, float* WT);
+// part: mpc_block_prec_partial_doubles(Bo, nrhs) doubles of scratch
 int mpc_apply_block_prec(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
                          const float* Winv, const float* WinvT, double* W, double* Y, double* Z,
-                         int64_t ld, int nrhs);
+                         int64_t ld, int nrhs, double* part);
+int64_t mpc_block_prec_partial_doubles(int Bo, int nrhs);
 inline int mpc_block_bo(int64_t n) { return n > 8192 ? 2048 : (n > 2048 ? 1024 : 256); }
 
 // host-side generators shared with mpc_matgen (RNG identical to the device one)
