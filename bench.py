@@ -39,6 +39,9 @@ def main():
     ap.add_argument("--syrk-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--syrk-only", action="store_true",
+                    help="only the trailing update at the first-step shape and a 1 GiB device "
+                         "copy of known size (for hardware-counter runs), one JSON line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only (the factor and PCG run on a single "
@@ -55,6 +58,13 @@ def main():
     h = S.Handle(0)
     stream = torch.cuda.Stream()
     L.check(h.lib.mpc_set_stream(h.h, C.c_void_p(stream.cuda_stream)), "mpc_set_stream")
+
+    if args.syrk_only:
+        sy = syrk_roofline(h, stream, n, nb, args.syrk_reps, C0=None)
+        cp = copy_reference(stream, args.syrk_reps)
+        print(json.dumps({"syrk": sy, "copy": cp}), flush=True)
+        h.close()
+        return
 
     log("generating A", n)
     A, lam = S.matgen_householder(h, n, cond=1e6, lam_max=1e6, seed=SEED_A)
@@ -95,35 +105,8 @@ def main():
     log(f"fp8: {ms8:.2f} ms its={pi8.iters} rr={rr8:.2e}; fp32: {ms32:.2f} ms its={pi32.iters} rr={rr32:.2e}")
 
     # ---- SYRK roofline at the first-step shape (m = n - nb, k = nb), E4M3 tiles
-    m, k, tr = n - nb, nb, nb
-    rng = np.random.default_rng(1)
-    P = torch.from_numpy(rng.standard_normal((k, m)).astype(np.float32)).cuda()
-    ntile = (m + tr - 1) // tr
-    pay, sc, pt = S.quantize_panel(h, P, [L.PREC_E4M3] * ntile, tr)
-    Cbuf = torch.empty(m, m, dtype=torch.float32, device="cuda")
-    Cbuf.copy_(A[:m, :m])
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-
-    def syrk_call():
-        return h.lib.mpc_syrk(h.h, C.c_void_p(Cbuf.data_ptr()), m, m, k, C.c_void_p(pay.data_ptr()),
-                              pay.stride(0), C.c_void_p(sc.data_ptr()), tr, C.c_void_p(pt.data_ptr()))
-
-    torch.cuda.synchronize()
-    for _ in range(2):
-        L.check(syrk_call(), "mpc_syrk")
-    torch.cuda.synchronize()
-    ev0.record(stream)
-    for _ in range(args.syrk_reps):
-        syrk_call()
-    ev1.record(stream)
-    ev1.synchronize()
-    syrk_ms = ev0.elapsed_time(ev1) / args.syrk_reps
-    syrk_flops = float(m) * (m + 1) * k
-    syrk_bytes = float(m) * (m + 1) / 2 * 8 + m * k * 1.0 + m * (k // 32)
-    syrk_gbs = syrk_bytes / (syrk_ms * 1e-3) / 1e9
-    syrk_tf = syrk_flops / (syrk_ms * 1e-3) / 1e12
-    del Cbuf, pay, sc, P
-    log(f"syrk m={m} k={k}: {syrk_ms:.3f} ms  {syrk_gbs:.0f} GB/s  {syrk_tf:.1f} TFLOP/s")
+    sy = syrk_roofline(h, stream, n, nb, args.syrk_reps, C0=A)
+    m, k, syrk_ms, syrk_gbs, syrk_tf = sy["m"], sy["k"], sy["ms"], sy["gbs"], sy["tflops"]
 
     # ---- CPU restatement on a bounded sample (rank 0, N = 1)
     cpu = None
@@ -170,6 +153,70 @@ def main():
     }
     print(json.dumps(out), flush=True)
     h.close()
+
+
+def syrk_roofline(h, stream, n, nb, reps, C0=None):
+    """One trailing update at the first-step shape (m = n - nb, k = nb) with
+    all panel tiles in E4M3, timed with HIP events on the stream the kernel
+    runs on.  Algorithmic bytes: the lower triangle of C read and written in
+    FP32 (8 B per element incl. the diagonal), the E4M3 panel and its scales."""
+    import ctypes as C
+    import numpy as np
+    import torch
+    from cholesky_gpu_amd import _lib as L
+    from cholesky_gpu_amd import solver as S
+    m, k, tr = n - nb, nb, nb
+    rng = np.random.default_rng(1)
+    P = torch.from_numpy(rng.standard_normal((k, m)).astype(np.float32)).cuda()
+    ntile = (m + tr - 1) // tr
+    pay, sc, pt = S.quantize_panel(h, P, [L.PREC_E4M3] * ntile, tr)
+    Cbuf = torch.empty(m, m, dtype=torch.float32, device="cuda")
+    if C0 is not None:
+        Cbuf.copy_(C0[:m, :m])
+    else:
+        Cbuf.normal_()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def call():
+        return h.lib.mpc_syrk(h.h, C.c_void_p(Cbuf.data_ptr()), m, m, k, C.c_void_p(pay.data_ptr()),
+                              pay.stride(0), C.c_void_p(sc.data_ptr()), tr, C.c_void_p(pt.data_ptr()))
+
+    torch.cuda.synchronize()
+    for _ in range(2):
+        L.check(call(), "mpc_syrk")
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(reps):
+        call()
+    ev1.record(stream)
+    ev1.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    flops = float(m) * (m + 1) * k
+    nbytes = float(m) * (m + 1) / 2 * 8 + m * k * 1.0 + m * (k // 32)
+    del Cbuf, pay, sc, P
+    out = {"m": m, "k": k, "ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9,
+           "tflops": flops / (ms * 1e-3) / 1e12}
+    log(f"syrk m={m} k={k}: {ms:.3f} ms  {out['gbs']:.0f} GB/s  {out['tflops']:.1f} TFLOP/s")
+    return out
+
+
+def copy_reference(stream, reps, nbytes=1 << 30):
+    """A plain device-to-device copy of known size (reads and writes nbytes),
+    the scale for the hardware byte counters."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda").normal_()
+    b = torch.empty_like(a)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        b.copy_(a)
+        ev0.record(stream)
+        for _ in range(reps):
+            b.copy_(a)
+        ev1.record(stream)
+    ev1.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    log(f"copy {nbytes >> 20} MiB: {ms:.3f} ms  {2 * nbytes / (ms * 1e-3) / 1e9:.0f} GB/s")
+    return {"bytes_each_way": nbytes, "ms": ms, "gbs": 2 * nbytes / (ms * 1e-3) / 1e9}
 
 
 def cpu_baseline(n, nb, eps, gpu_iters, cpu_steps, fi8, h, A, B):

@@ -49,7 +49,7 @@ class FactorInfo(C.Structure):
 
 class PcgOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxit", C.c_int), ("stop_rule", C.c_int),
-                ("x0_mode", C.c_int)]
+                ("x0_mode", C.c_int), ("graph", C.c_int), ("check_every", C.c_int)]
 
 
 class PcgInfo(C.Structure):

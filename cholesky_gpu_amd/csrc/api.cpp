@@ -700,6 +700,8 @@ extern "C" void mpc_pcg_opts_default(mpc_pcg_opts* o) {
     o->maxit = 500;
     o->stop_rule = 0;
     o->x0_mode = 0;
+    o->graph = 1;
+    o->check_every = 0;
 }
 
 namespace {
@@ -773,6 +775,7 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     mpc_pcg_opts def;
     if (!o) { mpc_pcg_opts_default(&def); o = &def; }
     MPC_CHECK(o->tol > 0 && o->maxit >= 0, "pcg: tol > 0 and maxit >= 0 required");
+    MPC_CHECK(o->check_every >= 0, "pcg: check_every must be >= 0");
     MPC_HIP(hipSetDevice(h->device));
     hipStream_t st = h->stream;
     int rc;
@@ -782,7 +785,9 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         std::max<size_t>((size_t)mpc_symv_partial_doubles(n, nrhs),
                          (size_t)mpc_block_prec_partial_doubles(mpc_block_bo(n), nrhs)),
         (size_t)nrhs * ((n + 2047) / 2048) + 64);
-    const size_t nbytes = sizeof(double) * (6 * vec + part + 5 * (size_t)nrhs + 64) + sizeof(int) * nrhs + 1024;
+    const int hmax = o->maxit + 1;
+    const size_t nbytes = sizeof(double) * (6 * vec + part + 6 * (size_t)nrhs + 2 * (size_t)hmax + 64) +
+                          sizeof(int) * 3 * (size_t)nrhs + 1024;
     if ((rc = mpc_ws_reserve(h->pcg, nbytes))) return rc;
     pcg_bufs b;
     double* base = (double*)h->pcg.ptr;
@@ -790,7 +795,12 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     b.part = b.Y + vec;
     b.rz = b.part + part; b.rz_new = b.rz + nrhs; b.pq = b.rz_new + nrhs; b.rr = b.pq + nrhs;
     b.bb = b.rr + nrhs;
-    b.active = (int*)(b.bb + nrhs + 8);
+    double* d_relres = b.bb + nrhs;
+    double* d_ahist = d_relres + nrhs;
+    double* d_bhist = d_ahist + hmax;
+    b.active = (int*)(d_bhist + hmax + 8);
+    int* d_cand = b.active + nrhs;
+    int* d_iters = d_cand + nrhs;
 
     hipEvent_t e0, e1;
     MPC_HIP(hipEventCreate(&e0));
@@ -817,8 +827,8 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.Z, n, b.rz, b.part))) return rc;
     if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.R, n, b.rr, b.part))) return rc;
 
-    std::vector<double> hb(nrhs), hrr(nrhs), hrz(nrhs), hpq(nrhs), hrzn(nrhs);
-    std::vector<int> act(nrhs, 1), iters(nrhs, 0);
+    std::vector<double> hb(nrhs), hrr(nrhs), hrz(nrhs);
+    std::vector<int> act(nrhs, 1), iters(nrhs, 0), cand(nrhs, 0);
     std::vector<double> relres(nrhs, 0.0), alphas, betas;
     MPC_HIP(hipMemcpyAsync(hb.data(), b.bb, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
     MPC_HIP(hipMemcpyAsync(hrr.data(), b.rr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
@@ -833,69 +843,121 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         relres[j] = res;
         if (res <= o->tol) act[j] = 0; else ++nactive;
     }
-    bool act_dirty = nactive != nrhs;
-    int k = 0;
-    while (nactive > 0 && k < o->maxit) {
-        ++k;
-        if (act_dirty) {
-            MPC_HIP(hipMemcpyAsync(b.active, act.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
-            act_dirty = false;
+    MPC_HIP(hipMemcpyAsync(b.active, act.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+    MPC_HIP(hipMemcpyAsync(d_cand, cand.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+    MPC_HIP(hipMemcpyAsync(d_iters, iters.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+
+    // One iteration for every active column; convergence is tested on the
+    // device (candidates freeze), so iterations run back to back and the host
+    // only looks every few of them.
+    auto body = [&]() -> int {
+        int r;
+        if ((r = op_apply(st, op, lda, n, nrhs, b.P, n, b.Q, n, b.part))) return r;
+        if ((r = mpc_launch_dot(st, n, nrhs, b.P, n, b.Q, n, b.pq, b.part))) return r;
+        if ((r = mpc_launch_pcg_update_xr(st, n, nrhs, dX, ldx, b.R, n, b.P, n, b.Q, n, b.rz, b.pq,
+                                          b.active)))
+            return r;
+        if ((r = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return r;
+        if ((r = mpc_launch_dot(st, n, nrhs, b.R, n, b.Z, n, b.rz_new, b.part))) return r;
+        if ((r = mpc_launch_dot(st, n, nrhs, b.R, n, b.R, n, b.rr, b.part))) return r;
+        if ((r = mpc_launch_pcg_update_p(st, n, nrhs, b.P, n, b.Z, n, b.rz_new, b.rz, b.active)))
+            return r;
+        return mpc_launch_pcg_status(st, nrhs, b.rz, b.rz_new, b.pq, b.rr, b.bb, o->tol,
+                                     o->stop_rule, b.active, d_cand, d_iters, d_relres, d_ahist,
+                                     d_bhist, hmax);
+    };
+    // the iteration as a graph: one launch instead of ~60 (no memset nodes)
+    struct graph_holder {
+        hipGraphExec_t exec = nullptr;
+        ~graph_holder() { if (exec) (void)hipGraphExecDestroy(exec); }
+    } gh;
+    if (o->graph && nactive > 0 && o->maxit > 0) {
+        hipGraph_t g = nullptr;
+        MPC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int brc = body();
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        if (brc) { if (g) (void)hipGraphDestroy(g); return brc; }
+        if (ce != hipSuccess) {
+            mpc_set_error("pcg: graph capture failed: %s", hipGetErrorString(ce));
+            return MPC_ERR_HIP;
         }
-        if ((rc = op_apply(st, op, lda, n, nrhs, b.P, n, b.Q, n, b.part))) return rc;
-        if ((rc = mpc_launch_dot(st, n, nrhs, b.P, n, b.Q, n, b.pq, b.part))) return rc;
-        if ((rc = mpc_launch_pcg_update_xr(st, n, nrhs, dX, ldx, b.R, n, b.P, n, b.Q, n, b.rz, b.pq,
-                                           b.active)))
-            return rc;
-        if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return rc;
-        if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.Z, n, b.rz_new, b.part))) return rc;
-        if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.R, n, b.rr, b.part))) return rc;
-        if ((rc = mpc_launch_pcg_update_p(st, n, nrhs, b.P, n, b.Z, n, b.rz_new, b.rz, b.active))) return rc;
-        MPC_HIP(hipMemcpyAsync(hrr.data(), b.rr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-        MPC_HIP(hipMemcpyAsync(hrzn.data(), b.rz_new, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-        MPC_HIP(hipMemcpyAsync(hpq.data(), b.pq, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-        // rz <- rz_new for active columns (device copy; inactive columns keep theirs)
-        if ((rc = mpc_launch_copy2d(st, nrhs, 1, b.rz_new, nrhs, b.rz, nrhs))) return rc;
+        const hipError_t ie = hipGraphInstantiate(&gh.exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+            gh.exec = nullptr;
+            mpc_set_error("pcg: graph instantiation failed: %s", hipGetErrorString(ie));
+            return MPC_ERR_HIP;
+        }
+    }
+    auto run_body = [&]() -> int {
+        if (!gh.exec) return body();
+        MPC_HIP(hipGraphLaunch(gh.exec, st));
+        return 0;
+    };
+
+    int done = 0, next = o->check_every > 0 ? o->check_every : 1;
+    double prev_res = 0.0;
+    int prev_done = 0;
+    for (int j = 0; j < nrhs; ++j) prev_res = std::max(prev_res, act[j] ? relres[j] : 0.0);
+    while (nactive > 0 && done < o->maxit) {
+        const int nbody = std::min(next, o->maxit - done);
+        for (int q = 0; q < nbody; ++q)
+            if ((rc = run_body())) return rc;
+        done += nbody;
+        MPC_HIP(hipMemcpyAsync(act.data(), b.active, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+        MPC_HIP(hipMemcpyAsync(cand.data(), d_cand, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+        MPC_HIP(hipMemcpyAsync(iters.data(), d_iters, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+        MPC_HIP(hipMemcpyAsync(relres.data(), d_relres, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
         MPC_HIP(hipStreamSynchronize(st));
-        if (act[0]) {
-            alphas.push_back(hrz[0] / hpq[0]);
-            betas.push_back(hrzn[0] / hrz[0]);
-        }
-        bool need_true = false;
-        std::vector<int> cand(nrhs, 0);
-        for (int j = 0; j < nrhs; ++j) {
-            if (!act[j]) continue;
-            iters[j] = k;
-            double res = o->stop_rule == 1 ? std::sqrt(std::fabs(hrzn[j])) / nb2[j] : std::sqrt(hrr[j]) / nb2[j];
-            relres[j] = res;
-            if (res <= o->tol) { cand[j] = 1; need_true = true; }
-        }
-        for (int j = 0; j < nrhs; ++j) hrz[j] = act[j] ? hrzn[j] : hrz[j];
-        if (need_true && o->stop_rule == 0) {
-            // confirm on the true residual B - A X; replace R where it lags
-            if ((rc = op_apply(st, op, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
-            if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.W, n))) return rc;
-            if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.W, n))) return rc;
-            if ((rc = mpc_launch_dot(st, n, nrhs, b.W, n, b.W, n, b.rr, b.part))) return rc;
-            MPC_HIP(hipMemcpyAsync(hrr.data(), b.rr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-            MPC_HIP(hipStreamSynchronize(st));
+        bool any_cand = false;
+        for (int j = 0; j < nrhs; ++j) any_cand |= cand[j] != 0;
+        if (any_cand) {
+            if (o->stop_rule == 0) {
+                // confirm on the true residual B - A X; replace R where it lags
+                if ((rc = op_apply(st, op, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
+                if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.W, n))) return rc;
+                if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.W, n))) return rc;
+                if ((rc = mpc_launch_dot(st, n, nrhs, b.W, n, b.W, n, b.rr, b.part))) return rc;
+                MPC_HIP(hipMemcpyAsync(hrr.data(), b.rr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+                MPC_HIP(hipStreamSynchronize(st));
+            }
             for (int j = 0; j < nrhs; ++j) {
                 if (!cand[j]) continue;
-                double tr = std::sqrt(hrr[j]) / nb2[j];
+                cand[j] = 0;
+                double tr = o->stop_rule == 0 ? std::sqrt(hrr[j]) / nb2[j] : relres[j];
                 relres[j] = tr;
                 if (tr <= o->tol) {
-                    act[j] = 0;
-                    --nactive;
-                    act_dirty = true;
+                    --nactive;                       // converged: stays frozen
                 } else {
-                    // residual replacement for column j
-                    MPC_HIP(hipMemcpyAsync(b.R + (size_t)j * n, b.W + (size_t)j * n, sizeof(double) * n,
-                                           hipMemcpyDeviceToDevice, st));
+                    act[j] = 1;                      // residual replacement, continue
+                    MPC_HIP(hipMemcpyAsync(b.R + (size_t)j * n, b.W + (size_t)j * n,
+                                           sizeof(double) * n, hipMemcpyDeviceToDevice, st));
                 }
             }
-        } else if (need_true) {
-            for (int j = 0; j < nrhs; ++j)
-                if (cand[j]) { act[j] = 0; --nactive; act_dirty = true; }
+            MPC_HIP(hipMemcpyAsync(b.active, act.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+            MPC_HIP(hipMemcpyAsync(d_cand, cand.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
         }
+        // next look: where the current convergence rate reaches tol (1..8)
+        double res = 0.0;
+        for (int j = 0; j < nrhs; ++j) if (act[j]) res = std::max(res, relres[j]);
+        if (o->check_every > 0) {
+            next = o->check_every;
+        } else if (res > 0 && prev_res > 0 && res < prev_res) {
+            const double rate = std::log(res / prev_res) / (double)(done - prev_done);
+            const double need = std::log(o->tol / res) / rate;
+            next = (int)std::min(8.0, std::max(1.0, std::floor(need)));
+        } else {
+            next = 2;
+        }
+        prev_res = res;
+        prev_done = done;
+    }
+    if (iters[0] > 0) {
+        const int m = std::min(iters[0], hmax);
+        alphas.resize(m);
+        betas.resize(m);
+        MPC_HIP(hipMemcpyAsync(alphas.data(), d_ahist, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+        MPC_HIP(hipMemcpyAsync(betas.data(), d_bhist, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     }
     MPC_HIP(hipEventRecord(e1, st));
     MPC_HIP(hipStreamSynchronize(st));
@@ -906,8 +968,7 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     mpc_pcg_info pi;
     std::memset(&pi, 0, sizeof pi);
     pi.iters = *std::max_element(iters.begin(), iters.end());
-    pi.converged = 0;
-    for (int j = 0; j < nrhs; ++j) pi.converged += act[j] ? 0 : 1;
+    pi.converged = nrhs - nactive;
     pi.relres = *std::max_element(relres.begin(), relres.end());
     pi.ms_total = ms;
     pi.kappa_est = 1.0;
@@ -1166,8 +1227,30 @@ extern "C" int mpc_syrk(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_
     MPC_CHECK(tile_rows >= 128 && tile_rows % 128 == 0, "syrk: tile_rows multiple of 128");
     MPC_CHECK(row_stride >= 4 * k, "syrk: row_stride must be >= 4*k");
     MPC_HIP(hipSetDevice(h->device));
-    return mpc_launch_syrk(h->stream, dC, m, ldc, (int)k, d_payload, row_stride, d_scales, tile_rows,
-                           d_prec_of_tile, nullptr);
+    // launch only the compute classes (and FP8 format pairs) that occur
+    const int64_t nt = (m + tile_rows - 1) / tile_rows;
+    std::vector<uint8_t> pt((size_t)nt);
+    MPC_HIP(hipMemcpyAsync(pt.data(), d_prec_of_tile, (size_t)nt, hipMemcpyDeviceToHost, h->stream));
+    MPC_HIP(hipStreamSynchronize(h->stream));
+    auto cls = [](int p, int q) {
+        if (p == MPC_PREC_FP32 || q == MPC_PREC_FP32) return MPC_PREC_FP32;
+        if (p == MPC_PREC_BF16 || q == MPC_PREC_BF16) return MPC_PREC_BF16;
+        if (p == MPC_PREC_FP16 || q == MPC_PREC_FP16) return MPC_PREC_FP16;
+        return MPC_PREC_E4M3;
+    };
+    unsigned cmask = 0, fmask = 0;
+    bool present[8] = {};
+    for (int64_t t = 0; t < nt; ++t) {
+        MPC_CHECK(pt[t] <= MPC_PREC_MXE5M2, "syrk: tile %lld has precision code %d", (long long)t, pt[t]);
+        present[pt[t]] = true;
+    }
+    for (int p = 0; p < 7; ++p)
+        for (int q = 0; q < 7; ++q)
+            if (present[p] && present[q]) cmask |= 1u << cls(p, q);
+    for (int p = MPC_PREC_E5M2; p <= MPC_PREC_MXE5M2; ++p)
+        if (present[p]) fmask |= (p == MPC_PREC_E4M3 || p == MPC_PREC_MXE4M3) ? 1u : 2u;
+    return mpc_launch_syrk_classes(h->stream, dC, m, ldc, (int)k, d_payload, row_stride, d_scales,
+                                   tile_rows, d_prec_of_tile, nullptr, cmask, fmask, 0, -1);
 }
 
 extern "C" int mpc_apply_precond(mpc_handle* h, const float* dL, int64_t ldl, int64_t n,

@@ -45,8 +45,10 @@ __global__ __launch_bounds__(256) void gp_grad_tile_kernel(
     for (int idx = tid; idx < d * GT; idx += 256) {
         const int c = idx / GT, p = idx % GT;
         const double s = inv_ls[n_ls == 1 ? 0 : c];
-        xi[c][p] = i0 + p < n ? (double)X[(i0 + p) * ldx + c] * s : 0.0;
-        xj[c][p] = j0 + p < n ? (double)X[(j0 + p) * ldx + c] * s : 0.0;
+        const int64_t ri = i0 + p < n ? i0 + p : n - 1, rj = j0 + p < n ? j0 + p : n - 1;
+        const double a = (double)X[ri * ldx + c] * s, b = (double)X[rj * ldx + c] * s;
+        xi[c][p] = i0 + p < n ? a : 0.0;
+        xj[c][p] = j0 + p < n ? b : 0.0;
     }
     // m[u][v] for i = i0 + tx + 16u, j = j0 + ty + 16v
     double m[4][4] = {};
@@ -56,11 +58,13 @@ __global__ __launch_bounds__(256) void gp_grad_tile_kernel(
             const int p = idx % GT, r = idx / GT;
             const bool okr = r0 + r < R;
             const int64_t gi = i0 + p, gj = j0 + p;
-            const int64_t o = (int64_t)(r0 + r) * ldu;
-            Ui[p * GLD + r] = okr && gi < n ? U[o + gi] : 0.0;
-            Vi[p * GLD + r] = okr && gi < n ? V[o + gi] : 0.0;
-            Uj[p * GLD + r] = okr && gj < n ? U[o + gj] : 0.0;
-            Vj[p * GLD + r] = okr && gj < n ? V[o + gj] : 0.0;
+            const int64_t gic = gi < n ? gi : n - 1, gjc = gj < n ? gj : n - 1;
+            const int64_t o = (int64_t)(okr ? r0 + r : R - 1) * ldu;
+            const double ui = U[o + gic], vi = V[o + gic], uj = U[o + gjc], vj = V[o + gjc];
+            Ui[p * GLD + r] = okr && gi < n ? ui : 0.0;
+            Vi[p * GLD + r] = okr && gi < n ? vi : 0.0;
+            Uj[p * GLD + r] = okr && gj < n ? uj : 0.0;
+            Vj[p * GLD + r] = okr && gj < n ? vj : 0.0;
         }
         __syncthreads();
         for (int r = 0; r < GRC; ++r) {

@@ -134,10 +134,13 @@ __global__ __launch_bounds__(256) void rbf_kernel(const float* __restrict__ X, i
         for (int idx = threadIdx.x; idx < RD * RT; idx += 256) {
             int p = idx / RD, c = idx % RD;
             int cc = c0 + c;
-            T s = cc < d ? inv_ls[n_ls == 1 ? 0 : cc] : T(0);
-            int64_t ri = i0 + p, rj = j0 + p;
-            xi[c][p] = (cc < d && ri < n) ? (T)X[ri * ldx + cc] * s : T(0);
-            xj[c][p] = (cc < d && rj < n) ? (T)X[rj * ldx + cc] * s : T(0);
+            const int ccl = cc < d ? cc : d - 1;
+            const T s = inv_ls[n_ls == 1 ? 0 : ccl];
+            const int64_t ri = i0 + p, rj = j0 + p;
+            const T a = (T)X[(ri < n ? ri : n - 1) * ldx + ccl] * s;
+            const T b = (T)X[(rj < n ? rj : n - 1) * ldx + ccl] * s;
+            xi[c][p] = (cc < d && ri < n) ? a : T(0);
+            xj[c][p] = (cc < d && rj < n) ? b : T(0);
         }
         __syncthreads();
 #pragma unroll

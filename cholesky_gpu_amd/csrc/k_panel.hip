@@ -55,14 +55,17 @@ __global__ __launch_bounds__(NW * 64) void trsm_stats_kernel(float* __restrict__
     const int64_t r0 = (int64_t)blockIdx.x * 32;
     const int64_t row = r0 + n;
     const bool valid = row < s;
+    const int64_t rowc = valid ? row : s - 1;     // loads from clamped rows, then select
 
     v16f acc[NO];
 #pragma unroll
     for (int o = 0; o < NO; ++o) {
         const int b = wave + NW * o;
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-            acc[o][i] = valid ? A21[row + (int64_t)(32 * b + sigma(i, h)) * lda] : 0.f;
+        for (int i = 0; i < 16; ++i) {
+            const float x = A21[rowc + (int64_t)(32 * b + sigma(i, h)) * lda];
+            acc[o][i] = valid ? x : 0.f;
+        }
     }
     float ss = 0.f, am = 0.f;
 #pragma unroll
@@ -73,8 +76,10 @@ __global__ __launch_bounds__(NW * 64) void trsm_stats_kernel(float* __restrict__
         for (int o = 0; o < NO; ++o) {
             const int bp = wave + NW * o;
 #pragma unroll
-            for (int t = 0; t < 16; ++t)
-                lv[o][t] = bp > b ? -Ljj[(32 * bp + n) + (int64_t)(32 * b + sigma(t, h)) * lda] : 0.f;
+            for (int t = 0; t < 16; ++t) {
+                const float x = Ljj[(32 * bp + n) + (int64_t)(32 * b + sigma(t, h)) * lda];
+                lv[o][t] = bp > b ? -x : 0.f;
+            }
         }
         if (b % NW == wave) {
             const int o = b / NW;
@@ -90,7 +95,8 @@ __global__ __launch_bounds__(NW * 64) void trsm_stats_kernel(float* __restrict__
 #pragma unroll
                 for (int t = 0; t < 16; ++t) {
                     const int q = sigma(t, h);
-                    const float a = q <= n ? -Ljj[(32 * b + n) + (int64_t)(32 * b + q) * lda] : 0.f;
+                    const float x = Ljj[(32 * b + n) + (int64_t)(32 * b + q) * lda];
+                    const float a = q <= n ? -x : 0.f;
                     rr = __builtin_amdgcn_mfma_f32_32x32x2f32(a, xb[t], rr, 0, 0, 0);
                 }
 #pragma unroll

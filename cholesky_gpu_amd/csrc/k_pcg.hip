@@ -184,6 +184,42 @@ __global__ __launch_bounds__(RB) void update_p_kernel(int64_t n, double* __restr
     P[j * ldp + r] = fma(beta, P[j * ldp + r], Z[j * ldz + r]);
 }
 
+// End of one PCG iteration, per right-hand side j that is still active:
+// record column 0's CG coefficients (Lanczos estimate), rz <- rz_new, count
+// the iteration, and test convergence (stop_rule 0: sqrt(rr)/||b||, the
+// recursive residual, confirmed later on the true residual; 1: the reference
+// rule sqrt(|r.z|)/||b||).  A column that meets tol becomes a candidate and is
+// frozen (active = 0) so later iterations leave it untouched.
+__global__ __launch_bounds__(256) void pcg_status_kernel(int nrhs, double* __restrict__ rz,
+                                                        const double* __restrict__ rz_new,
+                                                        const double* __restrict__ pq,
+                                                        const double* __restrict__ rr,
+                                                        const double* __restrict__ bb, double tol,
+                                                        int stop_rule, int* __restrict__ active,
+                                                        int* __restrict__ cand,
+                                                        int* __restrict__ iters,
+                                                        double* __restrict__ relres,
+                                                        double* __restrict__ ahist,
+                                                        double* __restrict__ bhist, int hmax) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= nrhs || !active[j]) return;
+    const int k = iters[j];
+    if (j == 0 && k < hmax) {
+        ahist[k] = rz[0] / pq[0];
+        bhist[k] = rz_new[0] / rz[0];
+    }
+    rz[j] = rz_new[j];
+    iters[j] = k + 1;
+    double nb = sqrt(bb[j]);
+    if (nb == 0.0) nb = 1.0;
+    const double res = stop_rule == 1 ? sqrt(fabs(rz_new[j])) / nb : sqrt(rr[j]) / nb;
+    relres[j] = res;
+    if (res <= tol) {
+        cand[j] = 1;
+        active[j] = 0;
+    }
+}
+
 __global__ __launch_bounds__(RB) void axpby_kernel(int64_t n, double a, const double* __restrict__ X,
                                                   int64_t ldx, double b, double* __restrict__ Y,
                                                   int64_t ldy) {
@@ -269,6 +305,17 @@ int mpc_launch_pcg_update_p(hipStream_t st, int64_t n, int nrhs, double* P, int6
                             const double* rz_old, const int* active) {
     hipLaunchKernelGGL(update_p_kernel, dim3(blocks_for(n), (unsigned)nrhs), dim3(RB), 0, st, n, P,
                        ldp, Z, ldz, rz_new, rz_old, active);
+    MPC_LAUNCH_CHECK();
+    return 0;
+}
+
+int mpc_launch_pcg_status(hipStream_t st, int nrhs, double* rz, const double* rz_new,
+                          const double* pq, const double* rr, const double* bb, double tol,
+                          int stop_rule, int* active, int* cand, int* iters, double* relres,
+                          double* ahist, double* bhist, int hmax) {
+    hipLaunchKernelGGL(pcg_status_kernel, dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st,
+                       nrhs, rz, rz_new, pq, rr, bb, tol, stop_rule, active, cand, iters, relres,
+                       ahist, bhist, hmax);
     MPC_LAUNCH_CHECK();
     return 0;
 }

@@ -85,10 +85,12 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
             float4 v[LB];
 #pragma unroll
             for (int u = 0; u < LB; ++u) {
+                // clamped, unconditional load (selected below): a load under a
+                // per-lane condition becomes a branch with its own wait
                 const int c = cbase + 4 * u + wave;
-                v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (c < bs && r4 < bs && r4 + 3 >= c)
-                    v[u] = *reinterpret_cast<const float4*>(A + r4 + (int64_t)c * lda);
+                const int cc = c < bs ? c : bs - 1, rc = r4 < bs ? r4 : bs - 4;
+                const float4 x = *reinterpret_cast<const float4*>(A + rc + (int64_t)cc * lda);
+                v[u] = (c < bs && r4 < bs) ? x : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
             for (int u = 0; u < LB; ++u) {
@@ -114,9 +116,8 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
 #pragma unroll
             for (int u = 0; u < LB; ++u) {
                 const int c = c0 + u;
-                float x = (r == c) ? 1.f : 0.f;
-                if (r < bs && c < bs && r >= c && rb < nsb) x = A[r + (int64_t)c * lda];
-                v[u] = x;
+                const float y = A[(r < bs ? r : bs - 1) + (int64_t)(c < bs ? c : bs - 1) * lda];
+                v[u] = (r < bs && c < bs && r >= c && rb < nsb) ? y : ((r == c) ? 1.f : 0.f);
             }
 #pragma unroll
             for (int u = 0; u < LB; ++u) {

@@ -136,19 +136,35 @@ __device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[2][2]
     uint32_t loff[2];
 #pragma unroll
     for (int a = 0; a < 2; ++a) loff[a] = (uint32_t)((32 * a + n) + 4 * h * ld) * 4u;
-    // all 64 loads first (one HBM latency), then subtract and store
+    // all 64 loads first (one HBM latency), then subtract and store.  Loads
+    // are unconditional (a per-lane conditional load becomes a branch with its
+    // own wait): interior quadrants read every element (the diagonal
+    // quadrant's upper part is a valid address, masked at the store); edge
+    // quadrants read from a clamped row and column.
     float v[2][2][16];
+    const bool interior = rlim >= 64 && clim >= 64;
+    if (interior) {
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < 2; ++b)
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int r = 32 * a + n, c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
-                const bool ok = r < rlim && c < clim && (!t.diag || r >= c);
-                const char* col = cbase + (int64_t)(32 * b + 8 * (i >> 2) + (i & 3)) * ld * 4;
-                v[a][b][i] = ok ? *reinterpret_cast<const float*>(col + loff[a]) : 0.f;
-            }
+                for (int i = 0; i < 16; ++i) {
+                    const char* col = cbase + (int64_t)(32 * b + 8 * (i >> 2) + (i & 3)) * ld * 4;
+                    v[a][b][i] = *reinterpret_cast<const float*>(col + loff[a]);
+                }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int r = 32 * a + n, c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
+                    const int rc = r < rlim ? r : rlim - 1, cc = c < clim ? c : clim - 1;
+                    v[a][b][i] = reinterpret_cast<const float*>(cbase)[rc + (int64_t)cc * ld];
+                }
+    }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll

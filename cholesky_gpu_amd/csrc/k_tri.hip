@@ -65,15 +65,27 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(gemm_batch g) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = v16f{};
     for (int k0 = klo; k0 < khi; k0 += 32) {
-        for (int idx = tid; idx < T * 32; idx += 256) {
-            int mm = idx % T, kk = idx / T;
-            int gm = m0 + mm, gk = k0 + kk;
+        // all staging loads issued together from clamped (in-bounds) addresses,
+        // masked by a select afterwards (conditional loads would serialise)
+        constexpr int NL = T * 32 / 256;
+        float xa[NL], xb[NL];
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int idx = tid + 256 * u;
+            const int gm = m0 + idx % T, gk = k0 + idx / T;
+            xa[u] = A[(gm < g.m ? gm : g.m - 1) + (int64_t)(gk < g.k ? gk : g.k - 1) * g.lda];
+            const int gk2 = k0 + (idx & 31), gn = n0 + (idx >> 5);
+            xb[u] = B[(gk2 < g.k ? gk2 : g.k - 1) + (int64_t)(gn < g.n ? gn : g.n - 1) * g.ldb];
+        }
+#pragma unroll
+        for (int u = 0; u < NL; ++u) {
+            const int idx = tid + 256 * u;
+            const int mm = idx % T, kk = idx / T, gm = m0 + mm, gk = k0 + kk;
             const bool aok = gm < g.m && gk < g.k && (!g.a_lower || gk <= gm);
-            As[kk][mm] = aok ? A[gm + (int64_t)gk * g.lda] : 0.f;
-            int k2 = idx & 31, nn = idx >> 5;
-            int gk2 = k0 + k2, gn = n0 + nn;
+            As[kk][mm] = aok ? xa[u] : 0.f;
+            const int k2 = idx & 31, nn = idx >> 5, gk2 = k0 + k2, gn = n0 + nn;
             const bool bok = gk2 < g.k && gn < g.n && (!g.b_lower || gk2 >= gn);
-            Bs[k2][nn] = bok ? B[gk2 + (int64_t)gn * g.ldb] : 0.f;
+            Bs[k2][nn] = bok ? xb[u] : 0.f;
         }
         __syncthreads();
         const int r = lane & 31, h = lane >> 5;
@@ -204,40 +216,68 @@ __global__ __launch_bounds__(64 * FU_WAVES) void fwd_update_kernel(
 // flight at once.  Partial sums go to part[(jb * nrhs + j) * bI + i];
 // tri_gemv_reduce adds the chunks of a row in order (deterministic).
 constexpr int TG = 64;
+template <bool UPPER>
 __global__ __launch_bounds__(64) void tri_gemv_tile_kernel(const float* __restrict__ M, int ld,
-                                                          int bI, int upper,
-                                                          const double* __restrict__ V,
+                                                          int bI, const double* __restrict__ V,
                                                           int64_t ldv, int nrhs,
                                                           double* __restrict__ part) {
-    const int nb = (bI + TG - 1) / TG;
     // tile t -> (ib, jb): lower ib >= jb, t = ib (ib + 1) / 2 + jb; upper mirrors it
     const int t = blockIdx.x;
     int a = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
     while (a * (a + 1) / 2 > t) --a;
     while ((a + 1) * (a + 2) / 2 <= t) ++a;
     const int b = t - a * (a + 1) / 2;
-    const int ib = upper ? b : a, jb = upper ? a : b;
-    (void)nb;
+    const int ib = UPPER ? b : a, jb = UPPER ? a : b;
     const int lane = threadIdx.x;
     const int i = ib * TG + lane;
-    const int ic = i < bI ? i : bI - 1;             // clamp: loads stay in bounds
     const int j0 = jb * TG;
-    const int jn = bI - j0 < TG ? bI - j0 : TG;
     const int r0 = blockIdx.y * NR;
     const int nj = nrhs - r0 < NR ? nrhs - r0 : NR;
+    // All 64 loads of the tile issued before any use.  A load guarded by a
+    // condition (or whose use is) would be sunk into a branch with its own
+    // wait, serialising the loads; hence three shapes of tile:
     float m[TG];
+    const bool full = j0 + TG <= bI && ib * TG + TG <= bI;
+    if (full && ib != jb) {                  // interior: no mask
+        const float* Mi = M + i + (int64_t)j0 * ld;
 #pragma unroll
-    for (int u = 0; u < TG; ++u) {
-        const int j = j0 + u;
-        const bool keep = u < jn && (upper ? j >= ic : j <= ic);
-        m[u] = keep ? M[ic + (int64_t)j * ld] : 0.f;
+        for (int u = 0; u < TG; ++u) m[u] = Mi[(int64_t)u * ld];
+    } else if (full) {                       // diagonal tile: per-lane triangle select
+        const float* Mi = M + i + (int64_t)j0 * ld;
+#pragma unroll
+        for (int u = 0; u < TG; ++u) m[u] = Mi[(int64_t)u * ld];
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            const bool keep = UPPER ? u >= lane : u <= lane;
+            m[u] = keep ? m[u] : 0.f;
+        }
+    } else {                                 // ragged last block: clamped addresses
+        const int ic = i < bI ? i : bI - 1;
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            const int j = j0 + u;
+            m[u] = M[ic + (int64_t)(j < bI ? j : bI - 1) * ld];
+        }
+#pragma unroll
+        for (int u = 0; u < TG; ++u) {
+            const int j = j0 + u;
+            const bool keep = j < bI && (UPPER ? j >= i : j <= i);
+            m[u] = keep ? m[u] : 0.f;
+        }
     }
     for (int r = 0; r < nj; ++r) {
         const double* v = V + (int64_t)(r0 + r) * ldv + j0;
         double acc = 0.0;
+        if (full) {
 #pragma unroll
-        for (int u = 0; u < TG; ++u)
-            if (u < jn) acc = fma((double)m[u], v[u], acc);
+            for (int u = 0; u < TG; ++u) acc = fma((double)m[u], v[u], acc);
+        } else {
+#pragma unroll
+            for (int u = 0; u < TG; ++u) {
+                const double vu = v[j0 + u < bI ? u : bI - 1 - j0];
+                acc = fma((double)m[u], vu, acc);
+            }
+        }
         if (i < bI) part[((int64_t)jb * nrhs + r0 + r) * bI + i] = acc;
     }
 }
@@ -273,17 +313,18 @@ __global__ __launch_bounds__(256) void bwd_update_kernel(const float* __restrict
     const float* col = L + p0 + c * ldl;
     float l[BU_MAXL];
 #pragma unroll
-    for (int u = 0; u < BU_MAXL; ++u) {
+    for (int u = 0; u < BU_MAXL; ++u) {      // clamped loads, then select (no branches)
         const int rr = lane + 64 * u;
-        l[u] = rr < bI ? col[rr] : 0.f;
+        const float x = col[rr < bI ? rr : bI - 1];
+        l[u] = rr < bI ? x : 0.f;
     }
     for (int j = 0; j < nj; ++j) {
         const double* x = X + p0 + (int64_t)(j0 + j) * ldx;
         double acc = 0.0;
 #pragma unroll
-        for (int u = 0; u < BU_MAXL; ++u) {
+        for (int u = 0; u < BU_MAXL; ++u) {      // l[u] = 0 beyond bI: clamped load, no branch
             const int rr = lane + 64 * u;
-            if (rr < bI) acc = fma((double)l[u], x[rr], acc);
+            acc = fma((double)l[u], x[rr < bI ? rr : bI - 1], acc);
         }
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
@@ -382,8 +423,13 @@ int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_
 static int tri_gemv(hipStream_t st, const float* M, int Bo, int bI, int upper, const double* V,
                     int64_t ldv, int nrhs, double* part, double* out, int64_t ldo) {
     const int nb = (bI + TG - 1) / TG;
-    hipLaunchKernelGGL(tri_gemv_tile_kernel, dim3((unsigned)(nb * (nb + 1) / 2), cdiv(nrhs, NR)),
-                       dim3(64), 0, st, M, Bo, bI, upper, V, ldv, nrhs, part);
+    const dim3 grid((unsigned)(nb * (nb + 1) / 2), cdiv(nrhs, NR));
+    if (upper)
+        hipLaunchKernelGGL(tri_gemv_tile_kernel<true>, grid, dim3(64), 0, st, M, Bo, bI, V, ldv,
+                           nrhs, part);
+    else
+        hipLaunchKernelGGL(tri_gemv_tile_kernel<false>, grid, dim3(64), 0, st, M, Bo, bI, V, ldv,
+                           nrhs, part);
     hipLaunchKernelGGL(tri_gemv_reduce_kernel, dim3(cdiv(bI, 256), (unsigned)nrhs), dim3(256), 0, st,
                        part, bI, nrhs, upper, out, ldo);
     MPC_LAUNCH_CHECK();

@@ -155,6 +155,10 @@ int mpc_launch_pcg_update_xr(hipStream_t st, int64_t n, int nrhs, double* X, int
 int mpc_launch_pcg_update_p(hipStream_t st, int64_t n, int nrhs, double* P, int64_t ldp,
                             const double* Z, int64_t ldz, const double* rz_new,
                             const double* rz_old, const int* active);
+int mpc_launch_pcg_status(hipStream_t st, int nrhs, double* rz, const double* rz_new,
+                          const double* pq, const double* rr, const double* bb, double tol,
+                          int stop_rule, int* active, int* cand, int* iters, double* relres,
+                          double* ahist, double* bhist, int hmax);
 int mpc_launch_axpby(hipStream_t st, int64_t n, int nrhs, double a, const double* X,
                      int64_t ldx, double b, double* Y, int64_t ldy);
 int mpc_launch_copy2d(hipStream_t st, int64_t n, int nrhs, const double* X, int64_t ldx,

@@ -76,9 +76,11 @@ def factor_opts(nb=256, update_prec=L.PREC_SWITCH, ladder=(L.PREC_E4M3, L.PREC_F
     return o
 
 
-def pcg_opts(tol=1e-10, maxit=500, stop_rule=0, x0_mode=0):
+def pcg_opts(tol=1e-10, maxit=500, stop_rule=0, x0_mode=0, graph=1, check_every=0):
     o = L.PcgOpts()
+    L.load().mpc_pcg_opts_default(C.byref(o))
     o.tol, o.maxit, o.stop_rule, o.x0_mode = tol, maxit, stop_rule, x0_mode
+    o.graph, o.check_every = graph, check_every
     return o
 
 

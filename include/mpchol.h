@@ -158,6 +158,9 @@ typedef struct {
     int    maxit;
     int    stop_rule;       /* 0: true residual (default); 1: reference sqrt(r.z)/||b|| */
     int    x0_mode;         /* 0: x0 = M^-1 b (reference); 1: x0 = 0; 2: x0 from dX */
+    int    graph;           /* 1 (default): replay each iteration as one HIP graph */
+    int    check_every;     /* iterations between host convergence checks;
+                               0 (default): adaptive, from the observed rate */
 } mpc_pcg_opts;
 typedef struct {
     int    iters;           /* max over right-hand sides */

