@@ -116,7 +116,7 @@ __global__ __launch_bounds__(NW * 64) void trsm_stats_kernel(float* __restrict__
             }
         }
         if (b + 1 < NBK) {
-            __syncthreads();
+            mpc_lds_barrier();
 #pragma unroll
             for (int o = 0; o < NO; ++o) {
                 const int bp = wave + NW * o;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(NW * 64) void trsm_stats_kernel(float* __restrict__
     // row statistics: lane (n, h) of every wave holds part of row n
     red_ss[wave][lane] = ss;
     red_am[wave][lane] = am;
-    __syncthreads();
+    mpc_lds_barrier();
     if (wave != 0) return;
     float tss = 0.f, tam = 0.f;
     if (lane < 32) {

@@ -5,9 +5,9 @@
 // micro_cholesky, micro_chol.hpp:14-44).  The block lives in LDS as 32x32
 // sub-tiles (lower triangle only, 36 sub-tiles x 32x33 floats = 152 KB for
 // bs = 256); the algorithm is right-looking over 32-column sub-panels:
-//   1. wave 0 factors the diagonal sub-tile and inverts it in one
-//      right-looking pass (row per lane, pivot column and inverse row
-//      broadcast through LDS);
+//   1. wave 0 factors the diagonal sub-tile (pipelined right-looking, row
+//      per lane) and inverts it (column per lane, rows of L broadcast from
+//      LDS);
 //   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32)
 //      with one refinement step against the factored sub-tile;
 //   3. the trailing sub-tiles receive the rank-32 update (MFMA 32x32x2 f32).
@@ -22,8 +22,13 @@ constexpr int SUBLD = 33;                 // padded sub-tile column stride
 constexpr int SUBSZ = 32 * SUBLD;         // floats per sub-tile
 constexpr int MAXSB = 8;                  // 256 / 32
 constexpr int NSUB = MAXSB * (MAXSB + 1) / 2;
+constexpr int LRLD = 36;                  // row stride of Lr (16-byte aligned rows)
 
 typedef float v16f __attribute__((ext_vector_type(16)));
+// LDS (address space 3) pointers: an opaque copy stays a 32-bit LDS address
+typedef const __attribute__((address_space(3))) float* lds_cfloat_p;
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(3))) f4* lds_cf4_p;
 
 __device__ __forceinline__ int sidx(int bi, int bj) { return bi * (bi + 1) / 2 + bj; }
 
@@ -58,7 +63,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
     __shared__ float T[NSUB * SUBSZ];
     __shared__ float Dinv[SUBSZ];
     __shared__ float colb[2][32];                   // column k of L, broadcast
-    __shared__ __align__(16) float xb[2][32];       // row k of L^{-1}, broadcast
+    __shared__ __align__(16) float Lr[33 * LRLD];   // row-major sub-tile; row 32: 1 / L_kk
     __shared__ int fail_col;
     __shared__ float red[256];
 
@@ -78,7 +83,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
     // block's columns; rows above the diagonal are read and dropped).
     const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 4 == 0) && (bs % 4 == 0);
     if (vec) {
-        constexpr int LB = 16;
+        constexpr int LB = 32;
         const int r4 = 4 * lane, rb = r4 >> 5;
         const int ncol = nsb * 32;
         for (int cbase = 0; cbase < ncol; cbase += 4 * LB) {
@@ -127,24 +132,32 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
             }
         }
     }
-    __syncthreads();
+    mpc_lds_barrier();
 
     for (int kb = 0; kb < nsb; ++kb) {
         float* D = T + sidx(kb, kb) * SUBSZ;
         // ---- 1. factor + invert the diagonal sub-tile (wave 0)
         if (wave == 0) {
-            // Factor and invert together, right-looking, row per lane: lane r
-            // holds row r of L (d) and of X = L^{-1} (xr).  Step k finishes
-            // column k of L and row k of X, broadcasts both through LDS (one
-            // write round, 16-byte reads), and applies the rank-1 updates.
-            const int r = lane & 31;
-            float d[32], xr[32];
+            // Pipelined right-looking factor, row per lane (lanes 32..63 mirror
+            // 0..31).  Column k+1 receives column k's update through one
+            // readlane (the pivot chain); the other columns receive it one step
+            // later from an LDS broadcast read right after it is written, so
+            // the read latency overlaps the next pivot.  The opaque copy
+            // of the lane id keeps the lane-dependent predicates inside the kb
+            // loop (hoisted out of it they exhaust the scalar registers).
+            int r = lane & 31;
+            asm volatile("" : "+v"(r));
+            // Updates are not predicated on r >= c: entries above the diagonal
+            // only accumulate values no valid entry ever reads (they are not
+            // broadcast, not pivots, and zeroed when the row is written back).
+            float d[32];
 #pragma unroll
             for (int c = 0; c < 32; ++c) {
-                d[c] = (c <= r) ? D[r + c * SUBLD] : 0.f;
-                xr[c] = (c == r) ? 1.f : 0.f;
+                const float v = D[r + c * SUBLD];
+                d[c] = (c <= r) ? v : 0.f;
             }
             int bad = -1;
+            float lc[32];          // column k-1 of L, prefetched from the broadcast
 #pragma unroll
             for (int k = 0; k < 32; ++k) {
                 float piv = rdlane(d[k], k);
@@ -153,51 +166,74 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
                     piv = 1.f;
                 }
                 const float l = sqrtf(piv);
-                if (r == k) d[k] = l;
-                else if (r > k) d[k] = d[k] / l;
-                if (r == k) {
-                    const float rl = 1.f / l;
-#pragma unroll
-                    for (int c = 0; c <= k; ++c) xr[c] *= rl;
-                }
+                float rl = __builtin_amdgcn_rcpf(l);  // one Newton step: rounded reciprocal
+                rl = fmaf(fmaf(-l, rl, 1.f), rl, rl);
+                d[k] = (r == k) ? l : d[k] * rl;
+                if (r == k) Lr[32 * LRLD + k] = rl;
                 if (k < 31) {
                     if (lane < 32) colb[k & 1][r] = d[k];
-                    if (lane == k) {
+                    const float v = rdlane(d[k], k + 1);   // l_{k+1,k}: the pivot chain
+                    d[k + 1] = fmaf(-d[k], v, d[k + 1]);
+                }
+                if (k >= 1) {                         // deferred: column k-1 -> columns > k
+                    const float lr = d[k - 1];
 #pragma unroll
-                        for (int c = 0; c <= k; c += 4)
-                            *reinterpret_cast<float4*>(&xb[k & 1][c]) =
-                                make_float4(xr[c], xr[c + 1], xr[c + 2], xr[c + 3]);
-                    }
-                    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                    __builtin_amdgcn_wave_barrier();
+                    for (int c = k + 1; c < 32; ++c) d[c] = fmaf(-lr, lc[c], d[c]);
+                }
+                if (k < 30) {                         // prefetch column k for step k+1
 #pragma unroll
-                    for (int c = k + 1; c < 32; ++c) {
-                        const float lck = colb[k & 1][c];
-                        if (r >= c) d[c] = fmaf(-d[k], lck, d[c]);
-                    }
-#pragma unroll
-                    for (int c = 0; c <= k; ++c) {
-                        const float xkc = xb[k & 1][c];
-                        if (r > k) xr[c] = fmaf(-d[k], xkc, xr[c]);
-                    }
+                    for (int c = k + 2; c < 32; ++c) lc[c] = colb[k & 1][c];
                 }
             }
-            // the whole sub-tile, zeros above the diagonal (d[c] = 0 for c > r):
-            // the trailing updates of earlier kb wrote the full 32x32 block,
-            // and the refinement below reads D as a lower-triangular operand
-            if (lane < 32) {
-                float* gd = dinv + (int64_t)((col0 >> 5) + kb) * 1024;
 #pragma unroll
-                for (int c = 0; c < 32; ++c) {
-                    D[r + c * SUBLD] = d[c];
-                    Dinv[r + c * SUBLD] = xr[c];
-                    if (kb * 32 < bs) gd[r + c * 32] = xr[c];
-                }
+            for (int c = 0; c < 32; ++c) d[c] = (c <= r) ? d[c] : 0.f;
+            // the whole sub-tile, zeros above the diagonal: the trailing
+            // updates of earlier kb wrote the full 32x32 block, and the
+            // refinement below reads D as a lower-triangular operand; Lr keeps
+            // a row-major copy for the inverse's broadcast reads
+            if (lane < 32) {
+#pragma unroll
+                for (int c = 0; c < 32; ++c) D[r + c * SUBLD] = d[c];
+#pragma unroll
+                for (int c = 0; c < 32; c += 4)
+                    *reinterpret_cast<float4*>(&Lr[r * LRLD + c]) =
+                        make_float4(d[c], d[c + 1], d[c + 2], d[c + 3]);
             }
             if (bad >= 0 && lane == 0 && kb * 32 + bad < bs && fail_col < 0)
                 fail_col = kb * 32 + bad;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            // inverse, column per lane: x[p] = (e_c[p] - sum_{q<p} L[p][q] x[q]) / L[p][p],
+            // row p of L read by every lane at the same LDS address (broadcast)
+            // (Lr lies beyond the 64 KB reach of the LDS instruction offset:
+            // an opaque base keeps every read at base + immediate offset)
+            const int c = r;
+            lds_cfloat_p lrb = (lds_cfloat_p)(Lr);
+            asm volatile("" : "+v"(lrb));
+            float x[32];
+#pragma unroll
+            for (int p = 0; p < 32; ++p) {
+                float a0 = (p == c) ? 1.f : 0.f, a1 = 0.f;
+#pragma unroll
+                for (int q4 = 0; q4 < p; q4 += 4) {
+                    const f4 lp = *(lds_cf4_p)(&lrb[p * LRLD + q4]);
+                    a0 = fmaf(-lp.x, x[q4], a0);
+                    if (q4 + 1 < p) a1 = fmaf(-lp.y, x[q4 + 1], a1);
+                    if (q4 + 2 < p) a0 = fmaf(-lp.z, x[q4 + 2], a0);
+                    if (q4 + 3 < p) a1 = fmaf(-lp.w, x[q4 + 3], a1);
+                }
+                x[p] = (a0 + a1) * lrb[32 * LRLD + p];
+            }
+            if (lane < 32) {
+                float* gd = dinv + (int64_t)((col0 >> 5) + kb) * 1024;
+#pragma unroll
+                for (int p = 0; p < 32; ++p) {
+                    Dinv[p + c * SUBLD] = x[p];
+                    if (kb * 32 < bs) gd[p + c * 32] = x[p];
+                }
+            }
         }
-        __syncthreads();
+        mpc_lds_barrier();
 
         // ---- 2. sub-tiles below: X = S * Dinv^T, refined once:
         //   R = S - X D^T, X += R Dinv^T  (D the factored diagonal sub-tile,
@@ -226,7 +262,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
 #pragma unroll
             for (int i = 0; i < 16; ++i) S[n + sig(i, h) * SUBLD] = x[i];
         }
-        __syncthreads();
+        mpc_lds_barrier();
 
         // ---- 3. trailing update: S(ib,jb) -= S(ib,kb) * S(jb,kb)^T, kb < jb <= ib
         {
@@ -257,7 +293,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
                 }
             }
         }
-        __syncthreads();
+        mpc_lds_barrier();
     }
 
     // ---- write back the lower triangle (thread = row), min pivot^2, failure
@@ -274,10 +310,10 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         }
     }
     red[tid] = pmin;
-    __syncthreads();
+    mpc_lds_barrier();
     for (int w = 128; w > 0; w >>= 1) {
         if (tid < w) red[tid] = fminf(red[tid], red[tid + w]);
-        __syncthreads();
+        mpc_lds_barrier();
     }
     if (tid == 0) {
         if (fail_col >= 0) {

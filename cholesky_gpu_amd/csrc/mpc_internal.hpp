@@ -92,6 +92,15 @@ struct mpc_handle {
     int          winv_bo = 0;
 };
 
+// Workgroup barrier that orders LDS accesses only: outstanding global loads
+// and stores are not waited for (a __syncthreads() fence waits for them),
+// for kernels that exchange data through LDS alone.
+__device__ __forceinline__ void mpc_lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // grow-only device allocation in a handle workspace
 int mpc_ws_reserve(mpc_workspace& w, size_t bytes);
 
