@@ -37,7 +37,7 @@ constexpr bool kTrsmRefine = true;
 // Dinv rows (lane m = lane % 32), B = X or acc columns (lane n = row of the
 // slab); accumulator register i of lane (n, h) holds column sig(i, h), row n.
 template <int NBK, int NW, bool REF>
-__global__ __launch_bounds__(NW * 64) void trsm_stats_kernel(float* __restrict__ A21, int64_t lda,
+__global__ __launch_bounds__(NW * 64, 2) void trsm_stats_kernel(float* __restrict__ A21, int64_t lda,
                                                          int64_t s, const float* __restrict__ dinv_j,
                                                          const float* __restrict__ Ljj,
                                                          float* __restrict__ rowss,

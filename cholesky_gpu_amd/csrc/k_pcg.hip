@@ -114,8 +114,19 @@ __global__ __launch_bounds__(256) void symv_reduce_kernel(int64_t n, int nt, int
     const int j = blockIdx.y;
     if (row >= n) return;
     const int64_t K = row / ST, r = row % ST;
+    // batches of 16 independent loads (clamped tile index, zero weight past
+    // the end), added in tile order
     double s = 0.0;
-    for (int64_t L = 0; L < nt; ++L) s += P[((K * nt + L) * nrhs + j) * ST + r];
+    for (int L0 = 0; L0 < nt; L0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int64_t L = L0 + u < nt ? L0 + u : nt - 1;
+            v[u] = P[((K * nt + L) * nrhs + j) * ST + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += L0 + u < nt ? v[u] : 0.0;
+    }
     Y[j * ldy + row] = s;
 }
 

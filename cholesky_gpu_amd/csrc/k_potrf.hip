@@ -10,7 +10,9 @@
 //      LDS);
 //   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32)
 //      with one refinement step against the factored sub-tile;
-//   3. the trailing sub-tiles receive the rank-32 update (MFMA 32x32x2 f32).
+//   3. the trailing sub-tiles receive the rank-32 update (MFMA 32x32x2 f32):
+//      sub-column kb+1 at once, the others while wave 0 factors sub-block
+//      kb+1 (they are disjoint from it).
 // Outputs: L in place (lower; the strict upper triangle of A is untouched),
 // dinv (column-major 32x32 inverse per sub-block), the running minimum pivot
 // squared and, on a non-positive pivot, the 1-based failing column.
@@ -53,6 +55,22 @@ __device__ __forceinline__ v16f mfma_32x32x32_lds(v16f acc, const float* a, int 
         acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
     }
     return acc;
+}
+
+// S(ib, jb) -= S(ib, kb) * S(jb, kb)^T for one 32x32 sub-tile (one wave, MFMA)
+__device__ __forceinline__ void update_subtile(float* T, int ib, int jb, int kb) {
+    const int lane = threadIdx.x & 63;
+    float* Cs = T + sidx(ib, jb) * SUBSZ;
+    const float* Ai = T + sidx(ib, kb) * SUBSZ;
+    const float* Bj = T + sidx(jb, kb) * SUBSZ;
+    const int n = lane & 31, h = lane >> 5;
+    v16f acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = Cs[sig(i, h) + n * SUBLD];
+    // A(m=r,k=q) = -Ai(r,q); B(k=q,n=c) = Bj(c,q)
+    acc = mfma_32x32x32_lds(acc, Ai, SUBLD, Bj, SUBLD, -1.f);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) Cs[sig(i, h) + n * SUBLD] = acc[i];
 }
 
 __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64_t lda, int bs,
@@ -136,7 +154,19 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
 
     for (int kb = 0; kb < nsb; ++kb) {
         float* D = T + sidx(kb, kb) * SUBSZ;
-        // ---- 1. factor + invert the diagonal sub-tile (wave 0)
+        // ---- 1. factor + invert the diagonal sub-tile (wave 0); meanwhile
+        // waves 1..3 finish the trailing update of kb-1 on the sub-columns
+        // beyond kb (3b below), which the factor of sub-block kb does not read
+        if (wave != 0 && kb >= 1) {
+            const int m = nsb - 1 - kb;                  // sub-columns kb+1 .. nsb-1
+            const int npair = m * (m + 1) / 2;
+            for (int pi = wave - 1; pi < npair; pi += 3) {
+                int ii = 0;
+                while ((ii + 1) * (ii + 2) / 2 <= pi) ++ii;
+                const int jj = pi - ii * (ii + 1) / 2;
+                update_subtile(T, kb + 1 + ii, kb + 1 + jj, kb - 1);
+            }
+        }
         if (wave == 0) {
             // Pipelined right-looking factor, row per lane (lanes 32..63 mirror
             // 0..31).  Column k+1 receives column k's update through one
@@ -264,35 +294,10 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         }
         mpc_lds_barrier();
 
-        // ---- 3. trailing update: S(ib,jb) -= S(ib,kb) * S(jb,kb)^T, kb < jb <= ib
-        {
-            const int nrem = nsb - 1 - kb;
-            const int npair = nrem * (nrem + 1) / 2;
-            for (int pi = wave; pi < npair; pi += 4) {
-                // decode pair index -> (ii, jj), jj <= ii, in units above kb
-                int ii = 0;
-                while ((ii + 1) * (ii + 2) / 2 <= pi) ++ii;
-                int jj = pi - ii * (ii + 1) / 2;
-                int ib = kb + 1 + ii, jb = kb + 1 + jj;
-                float* Cs = T + sidx(ib, jb) * SUBSZ;
-                const float* Ai = T + sidx(ib, kb) * SUBSZ;
-                const float* Bj = T + sidx(jb, kb) * SUBSZ;
-                const int n = lane & 31, h = lane >> 5;
-                v16f acc;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    int m = 8 * (i >> 2) + 4 * h + (i & 3);
-                    acc[i] = Cs[m + n * SUBLD];
-                }
-                // A(m=r,k=q) = -Ai(r,q); B(k=q,n=c) = Bj(c,q)
-                acc = mfma_32x32x32_lds(acc, Ai, SUBLD, Bj, SUBLD, -1.f);
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    int m = 8 * (i >> 2) + 4 * h + (i & 3);
-                    Cs[m + n * SUBLD] = acc[i];
-                }
-            }
-        }
+        // ---- 3a. trailing update of sub-column kb+1 (what sub-block kb+1
+        // needs next): S(ib,kb+1) -= S(ib,kb) S(kb+1,kb)^T, ib > kb.  The other
+        // sub-columns (3b) are updated during the next factor step.
+        for (int ib = kb + 1 + wave; ib < nsb; ib += 4) update_subtile(T, ib, kb + 1, kb);
         mpc_lds_barrier();
     }
 

@@ -291,8 +291,19 @@ __global__ __launch_bounds__(256) void tri_gemv_reduce_kernel(const double* __re
     if (i >= bI) return;
     const int nb = (bI + TG - 1) / TG, ib = i / TG;
     const int jlo = upper ? ib : 0, jhi = upper ? nb - 1 : ib;
+    // batches of 8 independent loads (clamped chunk index, zero weight beyond
+    // jhi), added in chunk order
     double s = 0.0;
-    for (int jb = jlo; jb <= jhi; ++jb) s += part[((int64_t)jb * nrhs + r) * bI + i];
+    for (int j0 = jlo; j0 <= jhi; j0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int jb = j0 + u <= jhi ? j0 + u : jhi;
+            v[u] = part[((int64_t)jb * nrhs + r) * bI + i];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += j0 + u <= jhi ? v[u] : 0.0;
+    }
     out[i + (int64_t)r * ldo] = s;
 }
 
