@@ -35,7 +35,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nb", type=int, default=256)
-    ap.add_argument("--eps", type=float, default=0.25)
+    ap.add_argument("--eps", type=float, default=0.125)
     ap.add_argument("--syrk-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=6)

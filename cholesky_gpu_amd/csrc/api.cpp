@@ -110,8 +110,11 @@ extern "C" int mpc_handle_create(mpc_handle** out, int device) {
     }
     h->own_stream = true;
     e = hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->wstream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_col, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wready, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wdone, hipEventDisableTiming);
     int rc = 0;
     if (e != hipSuccess) {
         mpc_set_error("stream/event creation failed: %s", hipGetErrorString(e));
@@ -134,12 +137,16 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->pstream) hipStreamSynchronize(h->pstream);
+    if (h->wstream) hipStreamSynchronize(h->wstream);
     for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
                              &h->pcg, &h->scratch, &h->winv, &h->gp64, &h->gp32, &h->gpvec})
         ws_free(*w);
     if (h->ev_col) hipEventDestroy(h->ev_col);
     if (h->ev_panel) hipEventDestroy(h->ev_panel);
+    if (h->ev_wready) hipEventDestroy(h->ev_wready);
+    if (h->ev_wdone) hipEventDestroy(h->ev_wdone);
     if (h->pstream) hipStreamDestroy(h->pstream);
+    if (h->wstream) hipStreamDestroy(h->wstream);
     if (h->own_stream) hipStreamDestroy(h->stream);
     delete h;
     return 0;
@@ -441,8 +448,11 @@ void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* 
 }
 
 // one factorisation attempt; A in place
+// W (optional): 2 nI Bo^2 floats receiving the big-block inverses of L (and
+// their transposes) for the PCG sweeps, built on wstream as each big diagonal
+// block of L becomes final, overlapping the rest of the factor.
 static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const mpc_factor_opts* o,
-                       double eps, uint8_t* map, float* tracker, mpc_factor_info* fi) {
+                       double eps, uint8_t* map, float* tracker, mpc_factor_info* fi, float* W) {
     const int nb = o->nb;
     const int64_t nt = (n + nb - 1) / nb;
     const int tr = tile_rows_for(nb);
@@ -499,6 +509,19 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                                        sm->tile_prec[b], sm, cmask, fmask, t0, t1);
     };
 
+    const int Bo = mpc_block_bo(n);
+    const int64_t nI = (n + Bo - 1) / Bo;
+    float* WT = W ? W + (size_t)nI * Bo * Bo : nullptr;
+    // after the panel of step s (on stream ps): build the inverse of the big
+    // block that step s completes, on wstream
+    auto build_w = [&](int64_t s, hipStream_t ps) -> int {
+        if (!W || !((((s + 1) * nb) % Bo) == 0 || s == nt - 1)) return 0;
+        const int64_t I = (s * nb) / Bo;
+        MPC_HIP(hipEventRecord(h->ev_wready, ps));
+        MPC_HIP(hipStreamWaitEvent(h->wstream, h->ev_wready, 0));
+        return mpc_build_block_inverses_range(h->wstream, dA, lda, n, Bo, dinv, W, WT, I, I + 1);
+    };
+
     const bool prof = o->profile != 0;
     double tp = 0, tt = 0, tq = 0, ts = 0;
     hipEvent_t e0, e1;
@@ -511,6 +534,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         // st.  Panel j+1 touches only block column j+1 and the other parity's
         // buffers; the rest of the update touches columns beyond it.
         if ((rc = panel(0, st))) return rc;
+        if ((rc = build_w(0, st))) return rc;
         for (int64_t j = 0; j + 1 < nt; ++j) {
             const int64_t bs1 = std::min<int64_t>(nb, n - (j + 1) * nb);
             const int64_t split = (bs1 + MPC_SYRK_TILE - 1) / MPC_SYRK_TILE;
@@ -518,6 +542,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
             MPC_HIP(hipEventRecord(h->ev_col, st));
             MPC_HIP(hipStreamWaitEvent(h->pstream, h->ev_col, 0));
             if ((rc = panel(j + 1, h->pstream))) return rc;
+            if ((rc = build_w(j + 1, h->pstream))) return rc;
             MPC_HIP(hipEventRecord(h->ev_panel, h->pstream));
             if ((rc = update(j, split, -1))) return rc;
             MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
@@ -563,6 +588,12 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
             hipEventElapsedTime(&ms, ev[3], ev[4]); ts += ms;
         }
         for (auto& e : ev) hipEventDestroy(e);
+        if (W && (rc = mpc_build_block_inverses_range(st, dA, lda, n, Bo, dinv, W, WT, 0, nI)))
+            return rc;
+    }
+    if (W && !prof) {
+        MPC_HIP(hipEventRecord(h->ev_wdone, h->wstream));
+        MPC_HIP(hipStreamWaitEvent(st, h->ev_wdone, 0));
     }
     MPC_HIP(hipEventRecord(e1, st));
     mpc_small hs;
@@ -619,7 +650,7 @@ static void flops_from_map(const uint8_t* map, int64_t n, int nb, mpc_factor_inf
 
 static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const mpc_factor_opts* o,
                        uint8_t* d_map, float* d_tracker, mpc_factor_info* info,
-                       const float* restore_from, int64_t restore_ld) {
+                       const float* restore_from, int64_t restore_ld, bool build_prec = false) {
     MPC_CHECK(h && dA, "factor: NULL argument");
     MPC_CHECK(n >= 1 && lda >= n, "factor: n=%lld lda=%lld", (long long)n, (long long)lda);
     mpc_factor_opts def;
@@ -652,11 +683,20 @@ static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         src = (const float*)h->amat.ptr;
         src_ld = n;
     }
+    // big-block inverses for a following PCG, built during the factor
+    float* W = nullptr;
+    const int Bo = mpc_block_bo(n);
+    if (build_prec) {
+        const int64_t nI = (n + Bo - 1) / Bo;
+        if ((rc = mpc_ws_reserve(h->winv, sizeof(float) * 2 * (size_t)nI * Bo * Bo))) return rc;
+        W = (float*)h->winv.ptr;
+    }
+    h->winv_owner = nullptr;
     double eps = o->eps_prime;
     mpc_factor_opts cur = *o;
     int attempt = 0;
     for (;;) {
-        rc = factor_once(h, dA, n, lda, &cur, eps, map, tracker, &fi);
+        rc = factor_once(h, dA, n, lda, &cur, eps, map, tracker, &fi, W);
         if (rc <= 0 || !o->fallback) break;
         // non-positive pivot: restore A, then retry with a smaller eps_prime
         // (twice), then with every update in FP32; an FP32 failure is final
@@ -679,11 +719,17 @@ static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         flops_from_map(hm.data(), n, nb, &fi);
     }
     if (info) *info = fi;
-    // the dinv blocks now describe this factor; big-block inverses are stale
-    h->winv_owner = nullptr;
+    // the dinv blocks now describe this factor, and so do the big-block
+    // inverses when they were built along with it
     h->dinv_owner = dA;
     h->dinv_n = n;
     h->dinv_ld = lda;
+    if (W && rc == 0) {
+        h->winv_owner = dA;
+        h->winv_n = n;
+        h->winv_ld = lda;
+        h->winv_bo = Bo;
+    }
     return rc;
 }
 
@@ -1039,7 +1085,7 @@ extern "C" int mpc_precond_cg(mpc_handle* h, const float* dA, int64_t lda, int64
     float* L = (float*)h->amat.ptr;
     if ((rc = mpc_launch_copy_f32(h->stream, n, n, dA, lda, L, n))) return rc;
     // the caller's A stays intact, so the fallback restores from it
-    rc = factor_impl(h, L, n, n, fo, nullptr, nullptr, finfo, dA, lda);
+    rc = factor_impl(h, L, n, n, fo, nullptr, nullptr, finfo, dA, lda, true);
     if (rc != 0) return rc;
     pcg_operator op;
     op.a32 = dA;
@@ -1143,7 +1189,7 @@ extern "C" int mpc_gp_loss_grad(mpc_handle* h, const float* dX, int64_t n, int d
     // 2. mixed-precision factor of the FP32 copy (the preconditioner)
     mpc_factor_info fi;
     std::memset(&fi, 0, sizeof fi);
-    rc = factor_impl(h, L, n, n, &o->factor, nullptr, nullptr, &fi, A32, n);
+    rc = factor_impl(h, L, n, n, &o->factor, nullptr, nullptr, &fi, A32, n, true);
     if (info) info->factor = fi;
     if (rc != 0) return rc;
     // 3. adjoint solve A [alpha, w_1..w_P] = [b, z_1..z_P] with the FP64 operator

@@ -25,7 +25,7 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ int sigma(int s, int h) { return 8 * (s >> 2) + 4 * h + (s & 3); }
 
 // one refinement step after the product with the explicit 32x32 inverse
-constexpr bool kTrsmRefine = true;
+constexpr bool kTrsmRefine = false;
 
 // One workgroup of NW waves per 32-row slab.  Wave w owns the column blocks
 // b = w, w + NW, ...  Step b: the owner finishes X_b = acc_b Dinv_b^T (one

@@ -25,6 +25,8 @@ constexpr int SUBSZ = 32 * SUBLD;         // floats per sub-tile
 constexpr int MAXSB = 8;                  // 256 / 32
 constexpr int NSUB = MAXSB * (MAXSB + 1) / 2;
 constexpr int LRLD = 36;                  // row stride of Lr (16-byte aligned rows)
+// one refinement step after the product with the explicit 32x32 inverse
+constexpr bool kPotrfRefine = false;
 
 typedef float v16f __attribute__((ext_vector_type(16)));
 // LDS (address space 3) pointers: an opaque copy stays a 32-bit LDS address
@@ -280,15 +282,17 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
             for (int t = 0; t < 16; ++t)
                 x = __builtin_amdgcn_mfma_f32_32x32x2f32(Dinv[n + sig(t, h) * SUBLD], s0[t], x,
                                                          0, 0, 0);
-            v16f r = s0;
+            if constexpr (kPotrfRefine) {
+                v16f r = s0;
 #pragma unroll
-            for (int t = 0; t < 16; ++t)
-                r = __builtin_amdgcn_mfma_f32_32x32x2f32(-D[n + sig(t, h) * SUBLD], x[t], r, 0, 0,
-                                                         0);
+                for (int t = 0; t < 16; ++t)
+                    r = __builtin_amdgcn_mfma_f32_32x32x2f32(-D[n + sig(t, h) * SUBLD], x[t], r, 0,
+                                                             0, 0);
 #pragma unroll
-            for (int t = 0; t < 16; ++t)
-                x = __builtin_amdgcn_mfma_f32_32x32x2f32(Dinv[n + sig(t, h) * SUBLD], r[t], x, 0,
-                                                         0, 0);
+                for (int t = 0; t < 16; ++t)
+                    x = __builtin_amdgcn_mfma_f32_32x32x2f32(Dinv[n + sig(t, h) * SUBLD], r[t], x,
+                                                             0, 0, 0);
+            }
 #pragma unroll
             for (int i = 0; i < 16; ++i) S[n + sig(i, h) * SUBLD] = x[i];
         }

@@ -18,6 +18,7 @@
 // triangles of A^-1 and C^-1.  The temporary B A^-1 lives in the unused upper
 // triangle of the Winv block.
 #include "mpc_internal.hpp"
+#include <algorithm>
 
 namespace {
 
@@ -119,8 +120,8 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(gemm_batch g) {
 
 // copy the 32x32 inverses onto the diagonal of the Winv blocks
 __global__ __launch_bounds__(256) void winv_init_kernel(const float* __restrict__ dinv, int64_t n,
-                                                       int Bo, float* __restrict__ W) {
-    const int64_t b = blockIdx.x;
+                                                       int Bo, float* __restrict__ W, int64_t b0) {
+    const int64_t b = b0 + blockIdx.x;
     const int64_t r0 = b * 32;
     const int64_t I = r0 / Bo;
     const int64_t p = r0 - I * Bo;
@@ -134,9 +135,10 @@ __global__ __launch_bounds__(256) void winv_init_kernel(const float* __restrict_
 
 // WT = Winv^T for each big block (lower part only is meaningful in Winv)
 __global__ __launch_bounds__(256) void transpose_blocks_kernel(const float* __restrict__ W,
-                                                              float* __restrict__ WT, int Bo) {
+                                                              float* __restrict__ WT, int Bo,
+                                                              int64_t I0) {
     __shared__ float t[32][33];
-    const int64_t I = blockIdx.z;
+    const int64_t I = I0 + blockIdx.z;
     const float* Wi = W + I * (int64_t)Bo * Bo;
     float* Ti = WT + I * (int64_t)Bo * Bo;
     const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
@@ -393,23 +395,31 @@ static int double_level(hipStream_t st, const float* L, int64_t ldl, int64_t n, 
     return launch_gemm(st, g2, nq * nblk);
 }
 
-int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
-                             const float* dinv, float* W, float* WT) {
+// Big blocks [I0, I1) only (each depends on L_II and the 32x32 inverses of
+// its diagonal alone, so blocks can be built as soon as they are factored).
+int mpc_build_block_inverses_range(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
+                                   const float* dinv, float* W, float* WT, int64_t I0,
+                                   int64_t I1) {
     if (Bo % 64 || Bo < 64) {
         mpc_set_error("block inverse: Bo must be a multiple of 64");
         return MPC_ERR_ARG;
     }
     const int64_t nI = (n + Bo - 1) / Bo;
     const int64_t nfull = n / Bo;                 // big blocks with Bo rows
-    const int64_t nb32 = (n + 31) / 32;
-    hipLaunchKernelGGL(winv_init_kernel, dim3((unsigned)nb32), dim3(256), 0, st, dinv, n, Bo, W);
+    if (I1 > nI) I1 = nI;
+    if (I0 >= I1) return 0;
+    const int64_t b0 = I0 * (Bo / 32), b1 = std::min<int64_t>((n + 31) / 32, I1 * (Bo / 32));
+    hipLaunchKernelGGL(winv_init_kernel, dim3((unsigned)(b1 - b0)), dim3(256), 0, st, dinv, n, Bo, W,
+                       b0);
     MPC_LAUNCH_CHECK();
+    const int64_t f1 = std::min(I1, nfull);      // full blocks in the range: [I0, f1)
     int rc;
     for (int h = 32; h < Bo; h *= 2) {
         const int pairs = Bo / (2 * h);
-        if (nfull > 0 && (rc = double_level(st, L, ldl, n, Bo, W, h, 0, (int)nfull, Bo, 0, pairs, h)))
+        if (f1 > I0 &&
+            (rc = double_level(st, L, ldl, n, Bo, W, h, I0, (int)(f1 - I0), Bo, 0, pairs, h)))
             return rc;
-        if (nI > nfull) {                          // ragged last big block
+        if (nI > nfull && nfull >= I0 && nfull < I1) {   // ragged last big block
             const int64_t I = nfull, rows = n - I * Bo;
             int npair = 0;
             for (int q = 0; q < pairs; ++q)
@@ -424,10 +434,15 @@ int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_
             }
         }
     }
-    dim3 tg((unsigned)(Bo / 32), (unsigned)(Bo / 32), (unsigned)nI);
-    hipLaunchKernelGGL(transpose_blocks_kernel, tg, dim3(256), 0, st, W, WT, Bo);
+    dim3 tg((unsigned)(Bo / 32), (unsigned)(Bo / 32), (unsigned)(I1 - I0));
+    hipLaunchKernelGGL(transpose_blocks_kernel, tg, dim3(256), 0, st, W, WT, Bo, I0);
     MPC_LAUNCH_CHECK();
     return 0;
+}
+
+int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
+                             const float* dinv, float* W, float* WT) {
+    return mpc_build_block_inverses_range(st, L, ldl, n, Bo, dinv, W, WT, 0, (n + Bo - 1) / Bo);
 }
 
 // z = L^{-T} L^{-1} w for nrhs columns.  W (in/out, destroyed), Y (work), Z (out)

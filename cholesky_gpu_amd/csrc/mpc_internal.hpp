@@ -68,6 +68,11 @@ struct mpc_handle {
     hipStream_t  pstream = nullptr;
     hipEvent_t   ev_col = nullptr;     // next block column updated (on stream)
     hipEvent_t   ev_panel = nullptr;   // next panel ready (on pstream)
+    // big-block inverses of L for the PCG sweeps, built on wstream during the
+    // factor as each big diagonal block becomes final
+    hipStream_t  wstream = nullptr;
+    hipEvent_t   ev_wready = nullptr;  // a big block of L is final
+    hipEvent_t   ev_wdone = nullptr;   // all inverses built (on wstream)
     // factor workspace
     mpc_workspace panel;      // quantised panel payload, n * nb * 4 bytes
     mpc_workspace scales;     // n * nb/32 bytes
@@ -196,6 +201,8 @@ int mpc_launch_probes(hipStream_t st, double* Z, int64_t ldz, int64_t n, int npr
 int mpc_launch_convert_f8(hipStream_t st, const float* x, uint8_t* y, int64_t count, int prec);
 int mpc_launch_dinv_from_L(hipStream_t st, const float* L, int64_t ldl, int64_t n, float* dinv);
 // k_tri.hip
+int mpc_build_block_inverses_range(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
+                                   const float* dinv, float* W, float* WT, int64_t I0, int64_t I1);
 int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
                              const float* dinv, float* W, float* WT);
 // part: mpc_block_prec_partial_doubles(Bo, nrhs) doubles of scratch
