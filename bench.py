@@ -36,6 +36,10 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nb", type=int, default=256)
     ap.add_argument("--eps", type=float, default=0.125)
+    ap.add_argument("--update-mixed", type=int, default=0,
+                    help="1: bulk trailing update as one launch with all precision classes")
+    ap.add_argument("--ladder", default="e4m3,fp32",
+                    help="per-tile precision ladder of the mixed run, lowest first")
     ap.add_argument("--syrk-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=6)
@@ -69,9 +73,13 @@ def main():
     log("generating A", n)
     A, lam = S.matgen_householder(h, n, cond=1e6, lam_max=1e6, seed=SEED_A)
     B = torch.from_numpy(np.random.default_rng(SEED_B).standard_normal((1, n))).cuda()
-    fo8 = S.factor_opts(nb=nb, update_prec=L.PREC_SWITCH, ladder=(L.PREC_E4M3, L.PREC_FP32),
-                        eps_prime=args.eps, fallback=1)
-    fo32 = S.factor_opts(nb=nb, update_prec=L.PREC_FP32, ladder=(L.PREC_FP32,), fallback=0)
+    names = {"e4m3": L.PREC_E4M3, "e5m2": L.PREC_E5M2, "mxe4m3": L.PREC_MXE4M3,
+             "mxe5m2": L.PREC_MXE5M2, "bf16": L.PREC_BF16, "fp16": L.PREC_FP16, "fp32": L.PREC_FP32}
+    ladder = tuple(names[x] for x in args.ladder.split(","))
+    fo8 = S.factor_opts(nb=nb, update_prec=L.PREC_SWITCH, ladder=ladder, eps_prime=args.eps,
+                        fallback=1, update_mixed=args.update_mixed)
+    fo32 = S.factor_opts(nb=nb, update_prec=L.PREC_FP32, ladder=(L.PREC_FP32,), fallback=0,
+                         update_mixed=args.update_mixed)
     po = S.pcg_opts(tol=1e-10, maxit=500)
 
     def solve(fo):
@@ -116,7 +124,9 @@ def main():
         except Exception as e:  # the GPU result stands without the CPU estimate
             log("cpu baseline failed:", repr(e))
 
-    fp8_frac = fi8.syrk_flops[L.PREC_E4M3] / max(1.0, sum(fi8.syrk_flops))
+    tot_flops = max(1.0, sum(fi8.syrk_flops))
+    flop_frac = {L.PREC_NAMES[p]: round(fi8.syrk_flops[p] / tot_flops, 4)
+                 for p in range(len(L.PREC_NAMES)) if fi8.syrk_flops[p] > 0}
     out = {
         "metric": "time to 1e-10 FP64 true residual, n=16384, FP8/FP32 per-tile-switching "
                   "Cholesky-PCG (vs FP32-update factor PCG)",
@@ -134,14 +144,14 @@ def main():
         "config": {"workload": "synthetic conditioned SPD n=16384, cond 1e6, block 256, "
                                "E4M3 per-tile-scale trailing update with FP32 switching, "
                                "PCG to 1e-10",
-                   "n": n, "nb": nb, "eps_prime": args.eps},
+                   "n": n, "nb": nb, "eps_prime": args.eps, "ladder": args.ladder},
         "fp32_factor_ms": round(ms32, 3),
         "speedup_vs_fp32_factor": round(ms32 / ms8, 3),
         "pcg_iters": {"fp8": pi8.iters, "fp32": pi32.iters},
         "relres": {"fp8": rr8, "fp32": rr32},
         "factor_ms": {"fp8": round(fi8.ms_total, 3), "fp32": round(fi32.ms_total, 3)},
         "pcg_ms": {"fp8": round(pi8.ms_total, 3), "fp32": round(pi32.ms_total, 3)},
-        "fp8_flop_fraction": round(fp8_frac, 4),
+        "syrk_flop_fraction": flop_frac,
         "factor_tflops": {"fp8": round(fi8.flops_total / (fi8.ms_total * 1e-3) / 1e12, 2),
                           "fp32": round(fi32.flops_total / (fi32.ms_total * 1e-3) / 1e12, 2)},
         "roofline": {"kernel": "syrk_kernel<E4M3> (one trailing update, m=%d k=%d)" % (m, k),

@@ -115,6 +115,7 @@ extern "C" int mpc_handle_create(mpc_handle** out, int device) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wready, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wdone, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_bulk, hipEventDisableTiming);
     int rc = 0;
     if (e != hipSuccess) {
         mpc_set_error("stream/event creation failed: %s", hipGetErrorString(e));
@@ -145,6 +146,7 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     if (h->ev_panel) hipEventDestroy(h->ev_panel);
     if (h->ev_wready) hipEventDestroy(h->ev_wready);
     if (h->ev_wdone) hipEventDestroy(h->ev_wdone);
+    if (h->ev_bulk) hipEventDestroy(h->ev_bulk);
     if (h->pstream) hipStreamDestroy(h->pstream);
     if (h->wstream) hipStreamDestroy(h->wstream);
     if (h->own_stream) hipStreamDestroy(h->stream);
@@ -402,10 +404,12 @@ extern "C" void mpc_factor_opts_default(mpc_factor_opts* o) {
     o->eps_prime = 0.25;
     o->fallback = 1;
     o->profile = 0;
+    o->update_mixed = 0;
 }
 
 static int validate_factor_opts(const mpc_factor_opts* o) {
     MPC_CHECK(o->nb == 64 || o->nb == 128 || o->nb == 256, "factor: nb must be 64, 128 or 256");
+    MPC_CHECK(o->update_mixed == 0 || o->update_mixed == 1, "factor: update_mixed must be 0 or 1");
     if (o->update_prec == MPC_PREC_SWITCH) {
         MPC_CHECK(o->n_ladder >= 1 && o->n_ladder <= 4, "factor: n_ladder must be 1..4");
         bool f16 = false, other16 = false;
@@ -500,13 +504,14 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                                       payload + b * pstride, (int64_t)nb * 4, scales + b * sstride,
                                       &sm->nan_flag);
     };
-    // trailing update of panel j restricted to SYRK tile columns [t0, t1) of A22
-    auto update = [&](int64_t j, int64_t t0, int64_t t1) -> int {
+    // trailing update of panel j on stream us restricted to SYRK tile columns
+    // [t0, t1) of A22 (tiles of MPC_SYRK_TILE, or of 64 * mixed when mixed)
+    auto update = [&](int64_t j, int64_t t0, int64_t t1, hipStream_t us, int mixed) -> int {
         const int64_t e = std::min<int64_t>((j + 1) * nb, n), s = n - e;
         const int b = (int)(j & 1);
-        return mpc_launch_syrk_classes(st, dA + e + e * lda, s, lda, nb, payload + b * pstride,
+        return mpc_launch_syrk_classes(us, dA + e + e * lda, s, lda, nb, payload + b * pstride,
                                        (int64_t)nb * 4, scales + b * sstride, tr,
-                                       sm->tile_prec[b], sm, cmask, fmask, t0, t1);
+                                       sm->tile_prec[b], sm, cmask, fmask, t0, t1, mixed);
     };
 
     const int Bo = mpc_block_bo(n);
@@ -529,24 +534,36 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     MPC_HIP(hipEventCreate(&e1));
     MPC_HIP(hipEventRecord(e0, st));
     if (!prof) {
-        // one-deep look-ahead: once the first block column of A22 is updated,
-        // panel j+1 is factored on pstream while the rest of A22 is updated on
-        // st.  Panel j+1 touches only block column j+1 and the other parity's
-        // buffers; the rest of the update touches columns beyond it.
-        if ((rc = panel(0, st))) return rc;
-        if ((rc = build_w(0, st))) return rc;
+        // One-deep look-ahead.  pstream carries the critical chain: the update
+        // of block column j+1 by panel j (one mixed-class launch of small
+        // tiles), then POTRF, TRSM and quantisation of panel j+1.  st
+        // carries the rest of panel j's update (block columns beyond j+1),
+        // which overlaps that chain.  Ordering: the bulk update of j waits
+        // for panel j; the block-column update of j waits for the bulk
+        // update of j-1 (the last other writer of block column j+1).  The
+        // panel buffers of parity j&1 are rewritten by panel j+2, which
+        // pstream runs after the block-column update of j+1, itself after
+        // the bulk update of j.
+        hipStream_t ps = h->pstream;
+        MPC_HIP(hipEventRecord(h->ev_col, st));
+        MPC_HIP(hipStreamWaitEvent(ps, h->ev_col, 0));
+        MPC_HIP(hipStreamWaitEvent(h->wstream, h->ev_col, 0));
+        if ((rc = panel(0, ps))) return rc;
+        if ((rc = build_w(0, ps))) return rc;
+        MPC_HIP(hipEventRecord(h->ev_panel, ps));
         for (int64_t j = 0; j + 1 < nt; ++j) {
             const int64_t bs1 = std::min<int64_t>(nb, n - (j + 1) * nb);
             const int64_t split = (bs1 + MPC_SYRK_TILE - 1) / MPC_SYRK_TILE;
-            if ((rc = update(j, 0, split))) return rc;
-            MPC_HIP(hipEventRecord(h->ev_col, st));
-            MPC_HIP(hipStreamWaitEvent(h->pstream, h->ev_col, 0));
-            if ((rc = panel(j + 1, h->pstream))) return rc;
-            if ((rc = build_w(j + 1, h->pstream))) return rc;
-            MPC_HIP(hipEventRecord(h->ev_panel, h->pstream));
-            if ((rc = update(j, split, -1))) return rc;
-            MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
+            MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));              // panel j
+            if (j > 0) MPC_HIP(hipStreamWaitEvent(ps, h->ev_bulk, 0));    // bulk j-1
+            if ((rc = update(j, split, -1, st, o->update_mixed ? 2 : 0))) return rc;
+            MPC_HIP(hipEventRecord(h->ev_bulk, st));
+            if ((rc = update(j, 0, 2 * split, ps, 1))) return rc;
+            if ((rc = panel(j + 1, ps))) return rc;
+            if ((rc = build_w(j + 1, ps))) return rc;
+            MPC_HIP(hipEventRecord(h->ev_panel, ps));
         }
+        MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
     } else {
         // sequential with per-phase events
         hipEvent_t ev[5] = {};
@@ -574,7 +591,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                          (int64_t)nb * 4, scales + b * sstride, &sm->nan_flag)))
                     return rc;
                 MPC_HIP(hipEventRecord(ev[3], st));
-                if ((rc = update(j, 0, -1))) return rc;
+                if ((rc = update(j, 0, -1, st, 0))) return rc;
             } else {
                 MPC_HIP(hipEventRecord(ev[2], st));
                 MPC_HIP(hipEventRecord(ev[3], st));

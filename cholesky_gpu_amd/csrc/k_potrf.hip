@@ -8,8 +8,7 @@
 //   1. wave 0 factors the diagonal sub-tile (pipelined right-looking, row
 //      per lane) and inverts it (column per lane, rows of L broadcast from
 //      LDS);
-//   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32)
-//      with one refinement step against the factored sub-tile;
+//   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32);
 //   3. the trailing sub-tiles receive the rank-32 update (MFMA 32x32x2 f32):
 //      sub-column kb+1 at once, the others while wave 0 factors sub-block
 //      kb+1 (they are disjoint from it).
@@ -25,6 +24,13 @@ constexpr int SUBSZ = 32 * SUBLD;         // floats per sub-tile
 constexpr int MAXSB = 8;                  // 256 / 32
 constexpr int NSUB = MAXSB * (MAXSB + 1) / 2;
 constexpr int LRLD = 36;                  // row stride of Lr (16-byte aligned rows)
+// Four waves (one per SIMD) at about 200 registers each: the factor then
+// fits on a CU beside one workgroup of the concurrent trailing update.
+// Eight waves are faster alone but need a whole CU, which the update's
+// stream of workgroups rarely leaves free (the look-ahead's critical path
+// then waits for one).
+constexpr int NW = 4;
+constexpr int NT = 64 * NW;
 // one refinement step after the product with the explicit 32x32 inverse
 constexpr bool kPotrfRefine = false;
 
@@ -75,7 +81,7 @@ __device__ __forceinline__ void update_subtile(float* T, int ib, int jb, int kb)
     for (int i = 0; i < 16; ++i) Cs[sig(i, h) + n * SUBLD] = acc[i];
 }
 
-__global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64_t lda, int bs,
+__global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_t lda, int bs,
                                                     int col0, float* __restrict__ dinv,
                                                     mpc_small* __restrict__ sm, int ntiles_next,
                                                     uint32_t* __restrict__ st_amax,
@@ -89,7 +95,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (sm->info != 0) return;                       // an earlier step failed
-    for (int t = tid; t < ntiles_next; t += 256) {
+    for (int t = tid; t < ntiles_next; t += NT) {
         st_amax[t] = 0u;
         st_maxrs[t] = 0u;
     }
@@ -98,7 +104,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
 
     // ---- load lower sub-tiles (padding: identity beyond bs), with many loads
     // per thread in flight (one batch per HBM latency).  16-byte path: lane l
-    // of wave w reads rows 4l..4l+3 of columns w, w+4, ...; it needs a 16-byte
+    // of wave w reads rows 4l..4l+3 of columns w, w+NW, ...; it needs a 16-byte
     // aligned block with lda and bs multiples of 4 (reads stay inside the
     // block's columns; rows above the diagonal are read and dropped).
     const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 4 == 0) && (bs % 4 == 0);
@@ -106,20 +112,20 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         constexpr int LB = 32;
         const int r4 = 4 * lane, rb = r4 >> 5;
         const int ncol = nsb * 32;
-        for (int cbase = 0; cbase < ncol; cbase += 4 * LB) {
+        for (int cbase = 0; cbase < ncol; cbase += NW * LB) {
             float4 v[LB];
 #pragma unroll
             for (int u = 0; u < LB; ++u) {
                 // clamped, unconditional load (selected below): a load under a
                 // per-lane condition becomes a branch with its own wait
-                const int c = cbase + 4 * u + wave;
+                const int c = cbase + NW * u + wave;
                 const int cc = c < bs ? c : bs - 1, rc = r4 < bs ? r4 : bs - 4;
                 const float4 x = *reinterpret_cast<const float4*>(A + rc + (int64_t)cc * lda);
                 v[u] = (c < bs && r4 < bs) ? x : make_float4(0.f, 0.f, 0.f, 0.f);
             }
 #pragma unroll
             for (int u = 0; u < LB; ++u) {
-                const int c = cbase + 4 * u + wave, cb = c >> 5;
+                const int c = cbase + NW * u + wave, cb = c >> 5;
                 if (c >= ncol || rb >= nsb || rb < cb) continue;
                 const float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
@@ -134,9 +140,9 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         }
     } else {
         constexpr int LB = 32;
-        const int r = tid, rb = r >> 5;
+        const int r = tid & 255, rb = r >> 5, half = tid >> 8;
         const int ncol = nsb * 32;
-        for (int c0 = 0; c0 < ncol; c0 += LB) {
+        for (int c0 = half * LB; c0 < ncol; c0 += (NT / 256) * LB) {
             float v[LB];
 #pragma unroll
             for (int u = 0; u < LB; ++u) {
@@ -162,7 +168,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         if (wave != 0 && kb >= 1) {
             const int m = nsb - 1 - kb;                  // sub-columns kb+1 .. nsb-1
             const int npair = m * (m + 1) / 2;
-            for (int pi = wave - 1; pi < npair; pi += 3) {
+            for (int pi = wave - 1; pi < npair; pi += NW - 1) {
                 int ii = 0;
                 while ((ii + 1) * (ii + 2) / 2 <= pi) ++ii;
                 const int jj = pi - ii * (ii + 1) / 2;
@@ -245,6 +251,10 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
             float x[32];
 #pragma unroll
             for (int p = 0; p < 32; ++p) {
+                // row p's reads wait for x[p-2]: without this order the
+                // scheduler hoists all 528 reads of L to the top, which
+                // exhausts the registers and spills to scratch memory
+                if (p >= 2) asm volatile("" : "+v"(lrb) : "v"(x[p - 2]));
                 float a0 = (p == c) ? 1.f : 0.f, a1 = 0.f;
 #pragma unroll
                 for (int q4 = 0; q4 < p; q4 += 4) {
@@ -272,7 +282,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         //   zero above its diagonal).  Worked in the transposed domain: lane
         //   (n, h) register i holds X(n, sig(i, h)), the k index of MFMA step t
         //   is sig(t, h) on both operands, so no LDS round trip is needed.
-        for (int ib = kb + 1 + wave; ib < nsb; ib += 4) {
+        for (int ib = kb + 1 + wave; ib < nsb; ib += NW) {
             float* S = T + sidx(ib, kb) * SUBSZ;
             const int n = lane & 31, h = lane >> 5;
             v16f s0, x = {};
@@ -301,16 +311,17 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
         // ---- 3a. trailing update of sub-column kb+1 (what sub-block kb+1
         // needs next): S(ib,kb+1) -= S(ib,kb) S(kb+1,kb)^T, ib > kb.  The other
         // sub-columns (3b) are updated during the next factor step.
-        for (int ib = kb + 1 + wave; ib < nsb; ib += 4) update_subtile(T, ib, kb + 1, kb);
+        for (int ib = kb + 1 + wave; ib < nsb; ib += NW) update_subtile(T, ib, kb + 1, kb);
         mpc_lds_barrier();
     }
 
-    // ---- write back the lower triangle (thread = row), min pivot^2, failure
+    // ---- write back the lower triangle (row r, every (NT/256)-th column per
+    // thread), min pivot^2, failure
     float pmin = INFINITY;
     {
-        const int r = tid, rb = r >> 5;
+        const int r = tid & 255, rb = r >> 5, half = tid >> 8;
         if (r < bs) {
-            for (int c = 0; c <= r; ++c) {
+            for (int c = half; c <= r; c += NT / 256) {
                 const float v = T[sidx(rb, c >> 5) * SUBSZ + (r & 31) + (c & 31) * SUBLD];
                 A[r + (int64_t)c * lda] = v;
             }
@@ -318,7 +329,7 @@ __global__ __launch_bounds__(256) void potrf_kernel(float* __restrict__ A, int64
             pmin = d * d;
         }
     }
-    red[tid] = pmin;
+    if (tid < 256) red[tid] = pmin;
     mpc_lds_barrier();
     for (int w = 128; w > 0; w >>= 1) {
         if (tid < w) red[tid] = fminf(red[tid], red[tid + w]);
@@ -345,7 +356,7 @@ int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0, fl
         mpc_set_error("potrf: %d panel tiles exceed 1024", ntiles_next);
         return MPC_ERR_ARG;
     }
-    hipLaunchKernelGGL(potrf_kernel, dim3(1), dim3(256), 0, st, A, lda, bs, col0, dinv, sm,
+    hipLaunchKernelGGL(potrf_kernel, dim3(1), dim3(NT), 0, st, A, lda, bs, col0, dinv, sm,
                        ntiles_next, stats_amax, stats_maxrs);
     MPC_LAUNCH_CHECK();
     return 0;

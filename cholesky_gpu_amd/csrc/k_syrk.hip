@@ -17,12 +17,14 @@
 //   FP32       v_mfma_f32_32x32x2f32 (lower formats expanded exactly)
 //
 // Launch: one workgroup (4 waves) per 128x128 lower tile of C, wave (wr, wc)
-// owns a 64x64 quadrant as 2x2 MFMA blocks.  The MFMA "row" index runs along
+// owns a 64x64 quadrant as 2x2 MFMA blocks (64x64 tiles of 1x1 blocks for the
+// look-ahead's block column).  The MFMA "row" index runs along
 // C's columns and the "column" index along C's rows, so each accumulator
 // register maps to 32 consecutive rows of the column-major C (128 B per
 // half-wave).  Block ids are remapped so consecutive tiles share an XCD.
 #include "mpc_internal.hpp"
 #include "mpc_formats.hpp"
+#include <type_traits>
 
 namespace {
 
@@ -31,8 +33,6 @@ typedef int v8i __attribute__((ext_vector_type(8)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef float v2f __attribute__((ext_vector_type(2)));
-
-constexpr int TILE = 128;
 
 __device__ __forceinline__ int pair_class(int p, int q) {
     if (p == MPC_PREC_FP32 || q == MPC_PREC_FP32) return MPC_PREC_FP32;
@@ -113,97 +113,125 @@ __device__ __forceinline__ v8h load8_f16(const operand_src& o, int64_t row, int 
 struct tile_ctx {
     float* C;
     int64_t ldc, m;
-    int64_t rowbase, colbase;   // first C row / col of this wave's 64x64 quadrant
+    int64_t rowbase, colbase;   // first C row / col of this wave's quadrant
     bool diag;                  // this quadrant straddles the diagonal (rowbase == colbase)
 };
 
-// Element (a, b, i) of the wave's quadrant sits at wave base + 32-bit offset
-//   (32a + n) + (32b + 8(i/4) + 4h + i%4) * ldc,
-// which stays below 2^31 for ldc <= 2^24; only the base pointer is 64-bit.
+// Each wave owns a (32 QA) x (32 QA) quadrant of C as QA x QA MFMA blocks.
+// Element (a, b, i) of the wave's quadrant sits at wave base + byte offset
+//   ((32a + n) + (32b + 8(i/4) + 4h + i%4) * ldc) * 4,
+// which stays below 2^31 for ldc <= 2^23; only the base pointer is 64-bit.
 // The product Q Q^T is accumulated from zero and subtracted from C once, as a
 // BLAS GEMM with beta = 1 does: seeding the accumulator with -C instead would
 // round every one of the k FMA steps at the scale of |C| rather than of the
 // partial sum, about sqrt(k) times the error for a trailing update.
-// Addresses: element (a, b, i) = wave base + column offset (32b + 8(i/4) +
-// i%4) * ldc, which is wave-uniform (scalar registers), + the lane's byte
-// offset (32a + n + 4h ldc) * 4, one 32-bit VGPR per a.
-__device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[2][2]) {
+template <int QA>
+__device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[QA][QA]) {
+    constexpr int QE = 32 * QA;
     const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
-    const char* cbase = reinterpret_cast<const char*>(t.C + t.rowbase + t.colbase * t.ldc);
-    char* wbase = reinterpret_cast<char*>(t.C + t.rowbase + t.colbase * t.ldc);
-    const int64_t ld = t.ldc;
+    // Buffer addressing: the wave's base in the resource (scalar), the
+    // column offset (32b + 8(i/4) + i%4) * ldc * 4 as the scalar soffset and
+    // the lane's (32a + n + 4h ldc) * 4 as the one 32-bit VGPR per a.  With
+    // global addressing every load would keep its own 64-bit VGPR address
+    // alive until the matching store.
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(t.C + t.rowbase + t.colbase * t.ldc, 0, 0x7fffffff,
+                                          0x00020000);
+    const int ld4 = (int)t.ldc * 4;
     const int rlim = (int)(t.m - t.rowbase), clim = (int)(t.m - t.colbase);
-    uint32_t loff[2];
+    int loff[QA];
 #pragma unroll
-    for (int a = 0; a < 2; ++a) loff[a] = (uint32_t)((32 * a + n) + 4 * h * ld) * 4u;
-    // all 64 loads first (one HBM latency), then subtract and store.  Loads
-    // are unconditional (a per-lane conditional load becomes a branch with its
+    for (int a = 0; a < QA; ++a) loff[a] = ((32 * a + n) + 4 * h * (int)t.ldc) * 4;
+    // all loads first (one HBM latency), then subtract and store.  Loads are
+    // unconditional (a per-lane conditional load becomes a branch with its
     // own wait): interior quadrants read every element (the diagonal
     // quadrant's upper part is a valid address, masked at the store); edge
     // quadrants read from a clamped row and column.
-    float v[2][2][16];
-    const bool interior = rlim >= 64 && clim >= 64;
+    float v[QA][QA][16];
+    const bool interior = rlim >= QE && clim >= QE;
     if (interior) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < QA; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < QA; ++b)
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const char* col = cbase + (int64_t)(32 * b + 8 * (i >> 2) + (i & 3)) * ld * 4;
-                    v[a][b][i] = *reinterpret_cast<const float*>(col + loff[a]);
-                }
+                for (int i = 0; i < 16; ++i)
+                    v[a][b][i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                        rs, loff[a], (32 * b + 8 * (i >> 2) + (i & 3)) * ld4, 0));
     } else {
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < QA; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < QA; ++b)
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
                     const int r = 32 * a + n, c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
                     const int rc = r < rlim ? r : rlim - 1, cc = c < clim ? c : clim - 1;
-                    v[a][b][i] = reinterpret_cast<const float*>(cbase)[rc + (int64_t)cc * ld];
+                    v[a][b][i] = __uint_as_float(
+                        __builtin_amdgcn_raw_buffer_load_b32(rs, rc * 4 + cc * ld4, 0, 0));
                 }
     }
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < QA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < QA; ++b)
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const int r = 32 * a + n, c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
                 const bool ok = r < rlim && c < clim && (!t.diag || r >= c);
-                char* col = wbase + (int64_t)(32 * b + 8 * (i >> 2) + (i & 3)) * ld * 4;
-                if (ok) *reinterpret_cast<float*>(col + loff[a]) = v[a][b][i] - acc[a][b][i];
+                if (ok)
+                    __builtin_amdgcn_raw_buffer_store_b32(
+                        __float_as_uint(v[a][b][i] - acc[a][b][i]), rs, loff[a],
+                        (32 * b + 8 * (i >> 2) + (i & 3)) * ld4, 0);
             }
 }
 
-// FP8 class: A operand = column-side rows, B operand = row-side rows.
-template <int FA, int FB>
-__device__ __forceinline__ void mm_fp8(const operand_src& oc, const operand_src& orow,
-                                       const tile_ctx& t, int k, v16f acc[2][2]) {
-    const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
-    int64_t ra[2], rb[2];
+// operand rows of the wave: column side (ra) and row side (rb), clamped to m
+template <int QA>
+__device__ __forceinline__ void operand_rows(const tile_ctx& t, int64_t ra[QA], int64_t rb[QA]) {
+    const int n = threadIdx.x & 31;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < QA; ++q) {
         ra[q] = t.colbase + 32 * q + n;
         if (ra[q] >= t.m) ra[q] = t.m - 1;
         rb[q] = t.rowbase + 32 * q + n;
         if (rb[q] >= t.m) rb[q] = t.m - 1;
     }
+}
+
+// body(kc) for kc = 0, STEP, 2 STEP, ... < k: fully unrolled when the extent
+// KC is a compile-time constant (every chunk's loads can then be issued
+// before the first product), a plain loop otherwise
+template <int KC, int STEP, typename F>
+__device__ __forceinline__ void k_loop(int k, F&& body) {
+    if constexpr (KC > 0) {
+#pragma unroll
+        for (int kc = 0; kc < KC; kc += STEP) body(kc);
+    } else {
+        for (int kc = 0; kc < k; kc += STEP) body(kc);
+    }
+}
+
+// FP8 class: A operand = column-side rows, B operand = row-side rows.
+template <int QA, int FA, int FB, int KC>
+__device__ __forceinline__ void mm_fp8(const operand_src& oc, const operand_src& orow,
+                                       const tile_ctx& t, int k, v16f acc[QA][QA]) {
+    const int h = (threadIdx.x & 63) >> 5;
+    int64_t ra[QA], rb[QA];
+    operand_rows<QA>(t, ra, rb);
     // Operand layout of the 8-bit 32x32x64 matrix-core op (measured on gfx950):
     // the scale of lane half h (E8M0 block h of the 64-wide K chunk) multiplies
     // bytes 0-15 of BOTH lane halves when h = 0 and bytes 16-31 of both when
     // h = 1.  So lane half h carries K [kc + 16h, +16) in bytes 0-15 and
     // K [kc + 32 + 16h, +16) in bytes 16-31; K block h then spans exactly the
     // 32 elements the hardware scales with lane half h's scale byte.
-    for (int kc = 0; kc < k; kc += 64) {
-        v8i A[2], B[2];
-        int sa[2], sb[2];
+    k_loop<KC, 64>(k, [&](int kc) {
+        v8i A[QA], B[QA];
+        int sa[QA], sb[QA];
         const int koff = kc + 32 * h;          // scale block of this lane half
         const int kl = kc + 16 * h;            // first K of bytes 0-15
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const uint8_t* pa = oc.payload + ra[q] * oc.row_stride + kl;
             const uint8_t* pb = orow.payload + rb[q] * orow.row_stride + kl;
             uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 32);
@@ -214,78 +242,104 @@ __device__ __forceinline__ void mm_fp8(const operand_src& oc, const operand_src&
             sb[q] = orow.scales[rb[q] * orow.kb + (koff >> 5)];
         }
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < QA; ++a)
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int b = 0; b < QA; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
                     A[b], B[a], acc[a][b], FA, FB, 0, sa[b], 0, sb[a]);
-    }
+    });
 }
 
-template <bool BF>
+template <int QA, bool BF, int KC>
 __device__ __forceinline__ void mm_16(const operand_src& oc, const operand_src& orow,
-                                      const tile_ctx& t, int k, v16f acc[2][2]) {
-    const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
-    int64_t ra[2], rb[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        ra[q] = t.colbase + 32 * q + n;
-        if (ra[q] >= t.m) ra[q] = t.m - 1;
-        rb[q] = t.rowbase + 32 * q + n;
-        if (rb[q] >= t.m) rb[q] = t.m - 1;
-    }
-    for (int kc = 0; kc < k; kc += 16) {
+                                      const tile_ctx& t, int k, v16f acc[QA][QA]) {
+    const int h = (threadIdx.x & 63) >> 5;
+    int64_t ra[QA], rb[QA];
+    operand_rows<QA>(t, ra, rb);
+    const int native = BF ? MPC_PREC_BF16 : MPC_PREC_FP16;
+    auto body = [&](int kc, auto both_native) {
         const int k0 = kc + 8 * h;
         if constexpr (BF) {
-            v8bf A[2], B[2];
+            v8bf A[QA], B[QA];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) { A[q] = load8_bf16(oc, ra[q], k0); B[q] = load8_bf16(orow, rb[q], k0); }
+            for (int q = 0; q < QA; ++q) {
+                if constexpr (decltype(both_native)::value) {
+                    A[q] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(
+                                                        oc.payload + ra[q] * oc.row_stride + 2 * k0));
+                    B[q] = __builtin_bit_cast(v8bf, *reinterpret_cast<const uint4*>(
+                                                        orow.payload + rb[q] * orow.row_stride + 2 * k0));
+                } else {
+                    A[q] = load8_bf16(oc, ra[q], k0);
+                    B[q] = load8_bf16(orow, rb[q], k0);
+                }
+            }
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < QA; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int b = 0; b < QA; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[b], B[a], acc[a][b], 0, 0, 0);
         } else {
-            v8h A[2], B[2];
+            v8h A[QA], B[QA];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) { A[q] = load8_f16(oc, ra[q], k0); B[q] = load8_f16(orow, rb[q], k0); }
+            for (int q = 0; q < QA; ++q) { A[q] = load8_f16(oc, ra[q], k0); B[q] = load8_f16(orow, rb[q], k0); }
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < QA; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int b = 0; b < QA; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A[b], B[a], acc[a][b], 0, 0, 0);
         }
-    }
+    };
+    // both operands stored in the class's own format: no per-load format branch
+    if (oc.prec == native && orow.prec == native)
+        k_loop<KC, 16>(k, [&](int kc) { body(kc, std::true_type{}); });
+    else
+        k_loop<0, 16>(k, [&](int kc) { body(kc, std::false_type{}); });
 }
 
+template <int QA, int KC>
 __device__ __forceinline__ void mm_f32(const operand_src& oc, const operand_src& orow,
-                                       const tile_ctx& t, int k, v16f acc[2][2]) {
-    const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
-    int64_t ra[2], rb[2];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        ra[q] = t.colbase + 32 * q + n;
-        if (ra[q] >= t.m) ra[q] = t.m - 1;
-        rb[q] = t.rowbase + 32 * q + n;
-        if (rb[q] >= t.m) rb[q] = t.m - 1;
-    }
+                                       const tile_ctx& t, int k, v16f acc[QA][QA]) {
+    const int h = (threadIdx.x & 63) >> 5;
+    int64_t ra[QA], rb[QA];
+    operand_rows<QA>(t, ra, rb);
     // k permutation: half h takes the 8-element groups k = 16c + 8h .. +8
-    for (int kc = 0; kc < k; kc += 16) {
+    auto body = [&](int kc, auto both_native) {
         const int k0 = kc + 8 * h;
-        float A[2][8], B[2][8];
+        float A[QA][8], B[QA][8];
 #pragma unroll
-        for (int q = 0; q < 2; ++q) { load8_f32(oc, ra[q], k0, A[q]); load8_f32(orow, rb[q], k0, B[q]); }
+        for (int q = 0; q < QA; ++q) {
+            if constexpr (decltype(both_native)::value) {
+                const float* pa = reinterpret_cast<const float*>(oc.payload + ra[q] * oc.row_stride) + k0;
+                const float* pb = reinterpret_cast<const float*>(orow.payload + rb[q] * orow.row_stride) + k0;
+                const float4 a0 = *reinterpret_cast<const float4*>(pa), a1 = *reinterpret_cast<const float4*>(pa + 4);
+                const float4 b0 = *reinterpret_cast<const float4*>(pb), b1 = *reinterpret_cast<const float4*>(pb + 4);
+                A[q][0] = a0.x; A[q][1] = a0.y; A[q][2] = a0.z; A[q][3] = a0.w;
+                A[q][4] = a1.x; A[q][5] = a1.y; A[q][6] = a1.z; A[q][7] = a1.w;
+                B[q][0] = b0.x; B[q][1] = b0.y; B[q][2] = b0.z; B[q][3] = b0.w;
+                B[q][4] = b1.x; B[q][5] = b1.y; B[q][6] = b1.z; B[q][7] = b1.w;
+            } else {
+                load8_f32(oc, ra[q], k0, A[q]);
+                load8_f32(orow, rb[q], k0, B[q]);
+            }
+        }
 #pragma unroll
         for (int e = 0; e < 8; ++e)
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < QA; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int b = 0; b < QA; ++b)
                     acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[b][e], B[a][e], acc[a][b], 0, 0, 0);
-    }
+    };
+    if (oc.prec == MPC_PREC_FP32 && orow.prec == MPC_PREC_FP32)
+        k_loop<KC, 16>(k, [&](int kc) { body(kc, std::true_type{}); });
+    else
+        k_loop<0, 16>(k, [&](int kc) { body(kc, std::false_type{}); });
 }
 
-template <int CLS, int FA, int FB>
+// CLS >= 0: only pairs of that compute class (and, for the FP8 class, of the
+// element-format pair FA/FB) are computed; the others exit at once.
+// CLS < 0: every pair, its class chosen per workgroup (one launch for all).
+template <int QA, int CLS, int FA, int FB, int KC>
 __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_t m, int64_t ldc,
                                                    int k, const uint8_t* __restrict__ payload,
                                                    int64_t row_stride,
@@ -294,6 +348,7 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
                                                    const uint8_t* __restrict__ prec_of_tile,
                                                    const mpc_small* __restrict__ sm, int64_t npair,
                                                    int64_t per_xcd, int64_t mt, int64_t tcol0) {
+    constexpr int TILE = 64 * QA;
     if (sm && sm->info != 0) return;
     // XCD-aware remap: hardware places block b on XCD b % 8
     const int64_t b = blockIdx.x;
@@ -313,10 +368,11 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
     operand_src orow, ocol;
     orow.prec = prec_of_tile[(ti * TILE) / tile_rows];
     ocol.prec = prec_of_tile[(tj * TILE) / tile_rows];
-    if (pair_class(orow.prec, ocol.prec) != CLS) return;   // another class's launch
-    if constexpr (CLS == MPC_PREC_E4M3) {                  // another format pair's launch
-        if ((mpc_dev_is_e4m3(ocol.prec) ? 0 : 1) != FA || (mpc_dev_is_e4m3(orow.prec) ? 0 : 1) != FB)
-            return;
+    const int cls = pair_class(orow.prec, ocol.prec);
+    const int fa = mpc_dev_is_e4m3(ocol.prec) ? 0 : 1, fb = mpc_dev_is_e4m3(orow.prec) ? 0 : 1;
+    if constexpr (CLS >= 0) {
+        if (cls != CLS) return;                              // another class's launch
+        if (CLS == MPC_PREC_E4M3 && (fa != FA || fb != FB)) return;   // another format pair
     }
 
     const int wave = threadIdx.x >> 6, wr = wave >> 1, wc = wave & 1;
@@ -327,8 +383,8 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
     t.C = C;
     t.ldc = ldc;
     t.m = m;
-    t.rowbase = ti * TILE + 64 * wr;
-    t.colbase = tj * TILE + 64 * wc;
+    t.rowbase = ti * TILE + 32 * QA * wr;
+    t.colbase = tj * TILE + 32 * QA * wc;
     t.diag = diag && wr == wc;     // only the diagonal quadrants need the r >= c mask
     if (t.rowbase >= m || t.colbase >= m) return;
     orow.payload = ocol.payload = payload;
@@ -336,35 +392,54 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
     orow.row_stride = ocol.row_stride = row_stride;
     orow.kb = ocol.kb = k >> 5;
 
-    v16f acc[2][2];
+    v16f acc[QA][QA];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < QA; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = v16f{};
-    if constexpr (CLS == MPC_PREC_E4M3) {
-        mm_fp8<FA, FB>(ocol, orow, t, k, acc);
-    } else if constexpr (CLS == MPC_PREC_BF16) {
-        mm_16<true>(ocol, orow, t, k, acc);
-    } else if constexpr (CLS == MPC_PREC_FP16) {
-        mm_16<false>(ocol, orow, t, k, acc);
+        for (int b = 0; b < QA; ++b) acc[a][b] = v16f{};
+    const int c = CLS >= 0 ? CLS : cls;
+    if (c == MPC_PREC_E4M3) {
+        if (CLS >= 0) mm_fp8<QA, FA, FB, KC>(ocol, orow, t, k, acc);
+        else if (fa == 0 && fb == 0) mm_fp8<QA, 0, 0, KC>(ocol, orow, t, k, acc);
+        else if (fa == 0) mm_fp8<QA, 0, 1, KC>(ocol, orow, t, k, acc);
+        else if (fb == 0) mm_fp8<QA, 1, 0, KC>(ocol, orow, t, k, acc);
+        else mm_fp8<QA, 1, 1, KC>(ocol, orow, t, k, acc);
+    } else if (c == MPC_PREC_BF16) {
+        mm_16<QA, true, KC>(ocol, orow, t, k, acc);
+    } else if (c == MPC_PREC_FP16) {
+        mm_16<QA, false, KC>(ocol, orow, t, k, acc);
     } else {
-        mm_f32(ocol, orow, t, k, acc);
+        mm_f32<QA, KC>(ocol, orow, t, k, acc);
     }
-    update_c(t, acc);
+    update_c<QA>(t, acc);
 }
 
 }  // namespace
 
-// class_mask: bit c set if pairs of compute class c (MPC_PREC_FP32/BF16/FP16/E4M3)
-// can occur; fp8_mask: bit 0 if E4M3-element tiles can occur, bit 1 if E5M2.
-// One launch per class (and FP8 element-format pair); pairs belonging to
-// another launch exit at once.
+// C(lower) -= Q Q^T restricted to the tile columns [tcol0, tcol1) of C.
+// mixed = 0: tiles of MPC_SYRK_TILE, one launch per compute class present in
+// class_mask (and per FP8 element-format pair in fp8_mask; bit 0 E4M3
+// elements, bit 1 E5M2), the pairs of other launches exiting at once.
+// mixed = 1 or 2: tiles of 64 * mixed in ONE launch in which each workgroup
+// takes its own pair's class, so the classes run side by side (the block
+// column on the critical path of the look-ahead uses 64-row tiles: a 128-row
+// tile per workgroup leaves most of the chip idle there).
 int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                             const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                             int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
-                            unsigned class_mask, unsigned fp8_mask, int64_t tcol0, int64_t tcol1) {
+                            unsigned class_mask, unsigned fp8_mask, int64_t tcol0, int64_t tcol1,
+                            int mixed) {
     if (m <= 0) return 0;
-    if (k % 64 || tile_rows % TILE) {
+    if (ldc > (int64_t(1) << 22)) {       // 32-bit buffer offsets of a 128-column tile
+        mpc_set_error("syrk: ldc %lld exceeds 2^22", (long long)ldc);
+        return MPC_ERR_ARG;
+    }
+    if (mixed < 0 || mixed > 2) {
+        mpc_set_error("syrk: mixed must be 0, 1 or 2 (got %d)", mixed);
+        return MPC_ERR_ARG;
+    }
+    const int TILE = mixed ? 64 * mixed : MPC_SYRK_TILE;
+    if (k % 64 || tile_rows % MPC_SYRK_TILE) {
         mpc_set_error("syrk: k (%d) must be a multiple of 64 and tile_rows (%d) of 128", k,
                       tile_rows);
         return MPC_ERR_ARG;
@@ -376,18 +451,31 @@ int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, in
     const int64_t npair = start(tcol1) - start(tcol0);
     const int64_t per_xcd = (npair + 7) / 8;
     dim3 grid((unsigned)(per_xcd * 8));
-#define MPC_SYRK_LAUNCH(CLS, FA, FB)                                                             \
-    hipLaunchKernelGGL((syrk_kernel<CLS, FA, FB>), grid, dim3(256), 0, st, C, m, ldc, k, payload, \
-                       row_stride, scales, tile_rows, prec_of_tile, sm, npair, per_xcd, mt, tcol0)
+#define MPC_SYRK_LAUNCH(QA, CLS, FA, FB, KC)                                                      \
+    hipLaunchKernelGGL((syrk_kernel<QA, CLS, FA, FB, KC>), grid, dim3(256), 0, st, C, m, ldc, k,  \
+                       payload, row_stride, scales, tile_rows, prec_of_tile, sm, npair, per_xcd, \
+                       mt, tcol0)
+    if (mixed == 1) {             // latency-bound: the K loop unrolled for the usual k
+        if (k == 256) MPC_SYRK_LAUNCH(1, -1, 0, 0, 256);
+        else if (k == 128) MPC_SYRK_LAUNCH(1, -1, 0, 0, 128);
+        else MPC_SYRK_LAUNCH(1, -1, 0, 0, 0);
+        MPC_LAUNCH_CHECK();
+        return 0;
+    }
+    if (mixed == 2) {
+        MPC_SYRK_LAUNCH(2, -1, 0, 0, 0);
+        MPC_LAUNCH_CHECK();
+        return 0;
+    }
     if (class_mask & (1u << MPC_PREC_E4M3)) {
         const bool e4 = fp8_mask & 1u, e5 = fp8_mask & 2u;
-        if (e4) MPC_SYRK_LAUNCH(MPC_PREC_E4M3, 0, 0);
-        if (e4 && e5) { MPC_SYRK_LAUNCH(MPC_PREC_E4M3, 0, 1); MPC_SYRK_LAUNCH(MPC_PREC_E4M3, 1, 0); }
-        if (e5) MPC_SYRK_LAUNCH(MPC_PREC_E4M3, 1, 1);
+        if (e4) MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 0, 0, 0);
+        if (e4 && e5) { MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 0, 1, 0); MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 1, 0, 0); }
+        if (e5) MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 1, 1, 0);
     }
-    if (class_mask & (1u << MPC_PREC_BF16)) MPC_SYRK_LAUNCH(MPC_PREC_BF16, 0, 0);
-    if (class_mask & (1u << MPC_PREC_FP16)) MPC_SYRK_LAUNCH(MPC_PREC_FP16, 0, 0);
-    if (class_mask & (1u << MPC_PREC_FP32)) MPC_SYRK_LAUNCH(MPC_PREC_FP32, 0, 0);
+    if (class_mask & (1u << MPC_PREC_BF16)) MPC_SYRK_LAUNCH(2, MPC_PREC_BF16, 0, 0, 0);
+    if (class_mask & (1u << MPC_PREC_FP16)) MPC_SYRK_LAUNCH(2, MPC_PREC_FP16, 0, 0, 0);
+    if (class_mask & (1u << MPC_PREC_FP32)) MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 0, 0, 0);
 #undef MPC_SYRK_LAUNCH
     MPC_LAUNCH_CHECK();
     return 0;
@@ -399,5 +487,5 @@ int mpc_launch_syrk(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
     const unsigned all = (1u << MPC_PREC_E4M3) | (1u << MPC_PREC_BF16) | (1u << MPC_PREC_FP16) |
                          (1u << MPC_PREC_FP32);
     return mpc_launch_syrk_classes(st, C, m, ldc, k, payload, row_stride, scales, tile_rows,
-                                   prec_of_tile, sm, all, 3u, 0, -1);
+                                   prec_of_tile, sm, all, 3u, 0, -1, 0);
 }

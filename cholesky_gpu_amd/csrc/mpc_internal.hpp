@@ -62,12 +62,13 @@ struct mpc_handle {
     int          device = 0;
     hipStream_t  stream = nullptr;
     bool         own_stream = false;
-    // look-ahead: the next panel (POTRF, TRSM, quantise) runs on pstream while
-    // the trailing update of the current panel runs on stream; both at the
-    // default priority
+    // look-ahead: the next block column's update and panel (POTRF, TRSM,
+    // quantise) run on pstream while the rest of the current trailing update
+    // runs on stream; both at the default priority
     hipStream_t  pstream = nullptr;
-    hipEvent_t   ev_col = nullptr;     // next block column updated (on stream)
+    hipEvent_t   ev_col = nullptr;     // factor start (on stream)
     hipEvent_t   ev_panel = nullptr;   // next panel ready (on pstream)
+    hipEvent_t   ev_bulk = nullptr;    // bulk update of a step done (on stream)
     // big-block inverses of L for the PCG sweeps, built on wstream during the
     // factor as each big diagonal block becomes final
     hipStream_t  wstream = nullptr;
@@ -150,7 +151,7 @@ int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, in
                             const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                             int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
                             unsigned class_mask, unsigned fp8_mask, int64_t tcol0 = 0,
-                            int64_t tcol1 = -1);
+                            int64_t tcol1 = -1, int mixed = 0);
 // compute classes / FP8 element formats that a factor configuration can produce
 void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* class_mask,
                     unsigned* fp8_mask);
