@@ -115,6 +115,7 @@ extern "C" int mpc_handle_create(mpc_handle** out, int device) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wready, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wdone, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wpre, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_bulk, hipEventDisableTiming);
     int rc = 0;
     if (e != hipSuccess) {
@@ -124,6 +125,15 @@ extern "C" int mpc_handle_create(mpc_handle** out, int device) {
     if (!rc) rc = mpc_ws_reserve(h->small, sizeof(mpc_small));
     if (rc) { mpc_handle_destroy(h); return rc; }
     *out = h;
+    return 0;
+}
+
+// stream s waits for a big-block inverse build still running on wstream
+static int wait_winv(mpc_handle* h, hipStream_t s) {
+    if (h->winv_pending) {
+        MPC_HIP(hipStreamWaitEvent(s, h->ev_wdone, 0));
+        h->winv_pending = false;
+    }
     return 0;
 }
 
@@ -146,6 +156,7 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     if (h->ev_panel) hipEventDestroy(h->ev_panel);
     if (h->ev_wready) hipEventDestroy(h->ev_wready);
     if (h->ev_wdone) hipEventDestroy(h->ev_wdone);
+    if (h->ev_wpre) hipEventDestroy(h->ev_wpre);
     if (h->ev_bulk) hipEventDestroy(h->ev_bulk);
     if (h->pstream) hipStreamDestroy(h->pstream);
     if (h->wstream) hipStreamDestroy(h->wstream);
@@ -462,6 +473,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     const int tr = tile_rows_for(nb);
     hipStream_t st = h->stream;
     int rc;
+    if ((rc = wait_winv(h, st))) return rc;       // it reads dinv and the factor
     if ((rc = mpc_ws_reserve(h->panel, 2 * (size_t)(n + 128) * nb * 4))) return rc;
     if ((rc = mpc_ws_reserve(h->scales, 2 * (size_t)(n + 128) * (nb / 32)))) return rc;
     if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nt * nb / 32 + 8) * 1024 * sizeof(float)))) return rc;
@@ -524,6 +536,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         const int64_t I = (s * nb) / Bo;
         MPC_HIP(hipEventRecord(h->ev_wready, ps));
         MPC_HIP(hipStreamWaitEvent(h->wstream, h->ev_wready, 0));
+        if (I == nI - 1) MPC_HIP(hipEventRecord(h->ev_wpre, h->wstream));
         return mpc_build_block_inverses_range(h->wstream, dA, lda, n, Bo, dinv, W, WT, I, I + 1);
     };
 
@@ -608,9 +621,9 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         if (W && (rc = mpc_build_block_inverses_range(st, dA, lda, n, Bo, dinv, W, WT, 0, nI)))
             return rc;
     }
-    if (W && !prof) {
+    if (W && !prof) {            // users of W wait for it (wait_winv)
         MPC_HIP(hipEventRecord(h->ev_wdone, h->wstream));
-        MPC_HIP(hipStreamWaitEvent(st, h->ev_wdone, 0));
+        h->winv_pending = true;
     }
     MPC_HIP(hipEventRecord(e1, st));
     mpc_small hs;
@@ -703,6 +716,7 @@ static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     // big-block inverses for a following PCG, built during the factor
     float* W = nullptr;
     const int Bo = mpc_block_bo(n);
+    if ((rc = wait_winv(h, h->stream))) return rc;
     if (build_prec) {
         const int64_t nI = (n + Bo - 1) / Bo;
         if ((rc = mpc_ws_reserve(h->winv, sizeof(float) * 2 * (size_t)nI * Bo * Bo))) return rc;
@@ -711,13 +725,14 @@ static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     h->winv_owner = nullptr;
     double eps = o->eps_prime;
     mpc_factor_opts cur = *o;
-    int attempt = 0;
+    int attempt = 0, rc2_wait = 0;
     for (;;) {
         rc = factor_once(h, dA, n, lda, &cur, eps, map, tracker, &fi, W);
         if (rc <= 0 || !o->fallback) break;
         // non-positive pivot: restore A, then retry with a smaller eps_prime
         // (twice), then with every update in FP32; an FP32 failure is final
         const bool all_fp32 = cur.update_prec == MPC_PREC_FP32;
+        if ((rc2_wait = wait_winv(h, h->stream))) return rc2_wait;   // before A is restored
         if (all_fp32) break;
         int rc2 = mpc_launch_copy_f32(h->stream, n, n, src, src_ld, dA, lda);
         if (rc2) return rc2;
@@ -775,23 +790,37 @@ struct pcg_bufs {
 };
 
 // z = L^-T L^-1 r  (all columns), blocked with the big-block inverses
+// (with an inverse build in flight: wait for all but the last big block
+// now and for the last one just before its stage)
 int apply_prec(mpc_handle* h, const float* L, int64_t ldl, int64_t n, int nb, int nrhs,
                const double* R, double* Z, pcg_bufs& b) {
     (void)nb;
     hipStream_t st = h->stream;
-    int rc = mpc_launch_copy2d(st, n, nrhs, R, n, b.W, n);
-    if (rc) return rc;
     const int64_t nI = (n + h->winv_bo - 1) / h->winv_bo;
     const float* Wv = (const float*)h->winv.ptr;
+    hipEvent_t late = nullptr;
+    if (h->winv_pending) {
+        MPC_HIP(hipStreamWaitEvent(st, h->ev_wpre, 0));
+        late = h->ev_wdone;
+        h->winv_pending = false;
+    }
     return mpc_apply_block_prec(st, L, ldl, n, h->winv_bo, Wv, Wv + (size_t)nI * h->winv_bo * h->winv_bo,
-                                b.W, b.Y, Z, n, nrhs, b.part);
+                                R, b.Y, Z, n, nrhs, b.part, late, nI - 1);
 }
 }  // namespace
 
 // 32x32 diagonal inverses and big-block inverses of L, cached per factor buffer
-static int prepare_prec(mpc_handle* h, const float* dL, int64_t ldl, int64_t n) {
+// keep_pending: a PCG about to apply the preconditioner waits for an
+// inverse build still in flight itself (stage by stage); every other user
+// waits here.
+static int prepare_prec(mpc_handle* h, const float* dL, int64_t ldl, int64_t n,
+                        bool keep_pending = false) {
     hipStream_t st = h->stream;
     int rc;
+    const bool cached = h->dinv_owner == dL && h->dinv_n == n && h->dinv_ld == ldl &&
+                        h->winv_owner == dL && h->winv_n == n && h->winv_ld == ldl &&
+                        h->winv_bo == mpc_block_bo(n);
+    if (!(cached && keep_pending) && (rc = wait_winv(h, st))) return rc;
     const int64_t nsb = (n + 31) / 32;
     if (h->dinv_owner != dL || h->dinv_n != n || h->dinv_ld != ldl) {
         if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nsb + 8) * 1024 * sizeof(float)))) return rc;
@@ -842,11 +871,11 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     MPC_HIP(hipSetDevice(h->device));
     hipStream_t st = h->stream;
     int rc;
-    if ((rc = prepare_prec(h, dL, ldl, n))) return rc;
+    if ((rc = prepare_prec(h, dL, ldl, n, true))) return rc;
     const size_t vec = (size_t)n * nrhs;
     const size_t part = std::max<size_t>(
         std::max<size_t>((size_t)mpc_symv_partial_doubles(n, nrhs),
-                         (size_t)mpc_block_prec_partial_doubles(mpc_block_bo(n), nrhs)),
+                         (size_t)mpc_block_prec_partial_doubles(n, mpc_block_bo(n), nrhs)),
         (size_t)nrhs * ((n + 2047) / 2048) + 64);
     const int hmax = o->maxit + 1;
     const size_t nbytes = sizeof(double) * (6 * vec + part + 6 * (size_t)nrhs + 2 * (size_t)hmax + 64) +
@@ -1097,8 +1126,9 @@ extern "C" int mpc_precond_cg(mpc_handle* h, const float* dA, int64_t lda, int64
     MPC_HIP(hipSetDevice(h->device));
     mpc_factor_opts def;
     if (!fo) { mpc_factor_opts_default(&def); fo = &def; }
-    int rc = mpc_ws_reserve(h->amat, sizeof(float) * (size_t)n * n);
+    int rc = wait_winv(h, h->stream);              // a build may still read amat
     if (rc) return rc;
+    if ((rc = mpc_ws_reserve(h->amat, sizeof(float) * (size_t)n * n))) return rc;
     float* L = (float*)h->amat.ptr;
     if ((rc = mpc_launch_copy_f32(h->stream, n, n, dA, lda, L, n))) return rc;
     // the caller's A stays intact, so the fallback restores from it
@@ -1174,6 +1204,7 @@ extern "C" int mpc_gp_loss_grad(mpc_handle* h, const float* dX, int64_t n, int d
     // workspaces: A64, the FP32 copy (factor source), the factor L, vectors
     if ((rc = mpc_ws_reserve(h->gp64, sizeof(double) * (size_t)n * n))) return rc;
     if ((rc = mpc_ws_reserve(h->gp32, sizeof(float) * (size_t)n * n))) return rc;
+    if ((rc = wait_winv(h, h->stream))) return rc;
     if ((rc = mpc_ws_reserve(h->amat, sizeof(float) * (size_t)n * n))) return rc;
     const int nout = (n_ls == 1 ? 1 : d) + 2;
     const size_t vecR = (size_t)n * R;
@@ -1325,14 +1356,14 @@ extern "C" int mpc_apply_precond(mpc_handle* h, const float* dL, int64_t ldl, in
     int rc = prepare_prec(h, dL, ldl, n);
     if (rc) return rc;
     const size_t vec = (size_t)n * nrhs;
-    const size_t part = (size_t)mpc_block_prec_partial_doubles(h->winv_bo, nrhs);
+    const size_t part = (size_t)mpc_block_prec_partial_doubles(n, h->winv_bo, nrhs);
     if ((rc = mpc_ws_reserve(h->scratch, sizeof(double) * (3 * vec + part)))) return rc;
     double* W = (double*)h->scratch.ptr;
     double* Y = W + vec;
     double* Z = Y + vec;
     double* P = Z + vec;
     hipStream_t st = h->stream;
-    if ((rc = mpc_launch_copy2d(st, n, nrhs, dR, ldr, W, n))) return rc;
+    if ((rc = mpc_launch_copy2d(st, n, nrhs, dR, ldr, W, n))) return rc;     // packed, ld n
     const int64_t nI = (n + h->winv_bo - 1) / h->winv_bo;
     const float* Wv = (const float*)h->winv.ptr;
     if ((rc = mpc_apply_block_prec(st, dL, ldl, n, h->winv_bo, Wv,
@@ -1347,7 +1378,9 @@ extern "C" int mpc_potrf_block(mpc_handle* h, float* dA, int n, int64_t lda, int
     MPC_CHECK(h && dA, "potrf_block: NULL argument");
     MPC_CHECK(n >= 1 && n <= 256 && lda >= n, "potrf_block: n must be 1..256");
     MPC_HIP(hipSetDevice(h->device));
-    int rc = mpc_ws_reserve(h->dinv, (size_t)16 * 1024 * sizeof(float));
+    int rc = wait_winv(h, h->stream);
+    if (rc) return rc;
+    rc = mpc_ws_reserve(h->dinv, (size_t)16 * 1024 * sizeof(float));
     if (rc) return rc;
     mpc_small* sm = (mpc_small*)h->small.ptr;
     hipLaunchKernelGGL(init_small_kernel, dim3(1), dim3(256), 0, h->stream, sm);

@@ -65,11 +65,13 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(gemm_batch g) {
     for (int a = 0; a < NB; ++a)
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = v16f{};
-    for (int k0 = klo; k0 < khi; k0 += 32) {
-        // all staging loads issued together from clamped (in-bounds) addresses,
-        // masked by a select afterwards (conditional loads would serialise)
-        constexpr int NL = T * 32 / 256;
-        float xa[NL], xb[NL];
+    // Staging: every load of a K step issued together from clamped
+    // (in-bounds) addresses and masked by a select when stored to LDS
+    // (conditional loads would serialise); the next step's loads are issued
+    // before this step's products, so their latency overlaps the MFMA work.
+    constexpr int NL = T * 32 / 256;
+    float xa[NL], xb[NL];
+    auto gload = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < NL; ++u) {
             const int idx = tid + 256 * u;
@@ -78,6 +80,9 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(gemm_batch g) {
             const int gk2 = k0 + (idx & 31), gn = n0 + (idx >> 5);
             xb[u] = B[(gk2 < g.k ? gk2 : g.k - 1) + (int64_t)(gn < g.n ? gn : g.n - 1) * g.ldb];
         }
+    };
+    if (klo < khi) gload(klo);
+    for (int k0 = klo; k0 < khi; k0 += 32) {
 #pragma unroll
         for (int u = 0; u < NL; ++u) {
             const int idx = tid + 256 * u;
@@ -89,6 +94,7 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(gemm_batch g) {
             Bs[k2][nn] = bok ? xb[u] : 0.f;
         }
         __syncthreads();
+        if (k0 + 32 < khi) gload(k0 + 32);
         const int r = lane & 31, h = lane >> 5;
 #pragma unroll
         for (int s = 0; s < 16; ++s) {
@@ -154,74 +160,164 @@ __global__ __launch_bounds__(256) void transpose_blocks_kernel(const float* __re
 }
 
 // ---------------------------------------------------------------- TRSV kernels
-constexpr int NR = 4;   // right-hand sides per pass
+// Both sweeps are left-looking: the update a big block receives from the
+// blocks solved before it is one GEMV over all of them, split into many
+// workgroups (partial sums per split, in a fixed order), and the partial sums
+// are folded into the right-hand side inside the triangular GEMV that
+// applies the block's inverse.  Per block: two or three launches, each wide
+// enough to fill the chip, instead of a narrow per-block update of every
+// later block.
+constexpr int NR = 4;       // right-hand sides per pass
+constexpr int TG = 64;      // tile edge of the triangular GEMV
+constexpr int FW_CW = 256;  // forward: columns per split (4 waves x 64)
+constexpr int BW_CPW = 8;   // backward: columns per wave (32 per workgroup)
+constexpr int BW_RW = 512;  // backward: rows per split (8 per lane)
 
-// W[r][j] -= sum_{c < cw} L[r][c0 + c] * Y[c0 + c][j]  for r in [r0, n)
-// 64 rows per workgroup (lane = row, 256 B per column load); 16 waves split
-// the columns into contiguous ranges; 32 independent loads per batch.
-constexpr int FU_WAVES = 16;
-constexpr int FU_UR = 32;
-__global__ __launch_bounds__(64 * FU_WAVES) void fwd_update_kernel(
-    const float* __restrict__ L, int64_t ldl, int64_t n, int64_t r0, int64_t c0, int cw,
-    const double* __restrict__ Y, int64_t ldy, double* __restrict__ W, int64_t ldw, int nrhs) {
-    __shared__ double red[FU_WAVES][NR][64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t r = r0 + (int64_t)blockIdx.x * 64 + lane;
-    const int j0 = blockIdx.y * NR;
+__device__ __forceinline__ double rdlane_d(double v, int l) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+// Forward, big block I = rows [p0, p0 + bI):
+//   part[(s nrhs + j) bI + i] = sum_{c in split s} L[p0 + i][c] Y[c][j],  c < p0
+// 64 rows per workgroup (lane = row: each column load is a 256-byte
+// segment), split s = FW_CW columns, wave w its 64 of them, all 64 loads of
+// a lane in flight before the first product; the split's Y values are staged
+// in LDS and read back by all lanes at once.
+__global__ __launch_bounds__(256) void fwd_gemv_partial_kernel(
+    const float* __restrict__ L, int64_t ldl, int64_t p0, int bI, const double* __restrict__ Y,
+    int64_t ldy, int nrhs, double* __restrict__ part) {
+    __shared__ double red[4][NR][64];
+    __shared__ double ys[NR][FW_CW];        // the split's Y values, read back broadcast
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const int ic = i < bI ? i : bI - 1;
+    const int sp = blockIdx.y;
+    const int j0 = blockIdx.z * NR;
     const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
-    const int per = (cw + FU_WAVES - 1) / FU_WAVES;
-    const int cb = wave * per, ce = cb + per < cw ? cb + per : cw;
+    const int64_t cs = (int64_t)sp * FW_CW;
+    const float* Lr = L + (p0 + ic);
+    float l[64];
+#pragma unroll
+    for (int u = 0; u < 64; ++u) {
+        const int64_t c = cs + wave * 64 + u;
+        const float x = Lr[(c < p0 ? c : p0 - 1) * ldl];     // clamped, then masked
+        l[u] = c < p0 ? x : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < NR * FW_CW / 256; ++q) {
+        const int idx = tid + 256 * q, j = idx / FW_CW, cc = idx % FW_CW;
+        const int64_t c = cs + cc;
+        const int jj = j < nj ? j : nj - 1;
+        const double v = Y[(int64_t)(j0 + jj) * ldy + (c < p0 ? c : p0 - 1)];
+        ys[j][cc] = (j < nj && c < p0) ? v : 0.0;
+    }
+    __syncthreads();
     double acc[NR];
 #pragma unroll
-    for (int j = 0; j < NR; ++j) acc[j] = 0.0;
-    const int64_t rr = r < n ? r : n - 1;          // clamp: loads stay in bounds
-    const float* Lr = L + rr + c0 * ldl;
-    const double* Yc = Y + c0 + (int64_t)j0 * ldy;
-    int c = cb;
-    for (; c + FU_UR <= ce; c += FU_UR) {
-        float l[FU_UR];
+    for (int j = 0; j < NR; ++j) {
+        double a0 = 0.0, a1 = 0.0;
+        if (j < nj) {
 #pragma unroll
-        for (int u = 0; u < FU_UR; ++u) l[u] = Lr[(int64_t)(c + u) * ldl];
-#pragma unroll
-        for (int j = 0; j < NR; ++j)
-            if (j < nj) {
-                const double* y = Yc + (int64_t)j * ldy + c;
-#pragma unroll
-                for (int u = 0; u < FU_UR; ++u) acc[j] = fma((double)l[u], y[u], acc[j]);
+            for (int u = 0; u < 64; u += 2) {
+                a0 = fma((double)l[u], ys[j][wave * 64 + u], a0);
+                a1 = fma((double)l[u + 1], ys[j][wave * 64 + u + 1], a1);
             }
-    }
-    for (; c < ce; ++c) {
-        float l0 = Lr[(int64_t)c * ldl];
-#pragma unroll
-        for (int j = 0; j < NR; ++j)
-            if (j < nj) acc[j] = fma((double)l0, Yc[(int64_t)j * ldy + c], acc[j]);
+        }
+        acc[j] = a0 + a1;
     }
 #pragma unroll
     for (int j = 0; j < NR; ++j) red[wave][j][lane] = acc[j];
     __syncthreads();
-    if (wave == 0 && r < n) {
+    if (wave == 0 && i < bI) {
 #pragma unroll
         for (int j = 0; j < NR; ++j)
-            if (j < nj) {
-                double s = 0.0;
+            if (j < nj)
+                part[((int64_t)sp * nrhs + j0 + j) * bI + i] =
+                    ((red[0][j][lane] + red[1][j][lane]) + red[2][j][lane]) + red[3][j][lane];
+    }
+}
+
+// Backward, big block I = columns [p0, p0 + bI), rows below it [p1, n):
+//   part[(s nrhs + j) bI + c] = sum_{r in split s} L[r][p0 + c] X[r][j]
+// 32 columns per workgroup (8 per wave), split s = BW_RW rows; lane k of a
+// wave holds rows lane + 64k of each of its columns (coalesced 256-byte
+// loads, 64 in flight), then the 64 lane sums of a column are added through
+// LDS: lane t sums 8 of column t/8's, three shuffle steps finish it.
+__global__ __launch_bounds__(256) void bwd_gemv_partial_kernel(
+    const float* __restrict__ L, int64_t ldl, int64_t p0, int bI, int64_t n,
+    const double* __restrict__ X, int64_t ldx, int nrhs, double* __restrict__ part) {
+    __shared__ double buf[4][BW_CPW][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t p1 = p0 + bI;
+    const int sp = blockIdx.y;
+    const int64_t rs = p1 + (int64_t)sp * BW_RW;
+    const int c0 = blockIdx.x * (4 * BW_CPW) + wave * BW_CPW;
+    const int j0 = blockIdx.z * NR;
+    const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
+    float l[BW_CPW][8];
 #pragma unroll
-                for (int w = 0; w < FU_WAVES; ++w) s += red[w][j][lane];
-                W[r + (int64_t)(j0 + j) * ldw] -= s;
+    for (int cc = 0; cc < BW_CPW; ++cc) {
+        const int c = c0 + cc < bI ? c0 + cc : bI - 1;
+        const float* col = L + (p0 + c) * ldl;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t r = rs + lane + 64 * k;
+            l[cc][k] = col[r < n ? r : n - 1];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        if (j < nj) {
+            const double* x = X + (int64_t)(j0 + j) * ldx;
+            double xv[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const int64_t r = rs + lane + 64 * k;
+                const double v = x[r < n ? r : n - 1];
+                xv[k] = r < n ? v : 0.0;                     // rows beyond n weigh nothing
             }
+#pragma unroll
+            for (int cc = 0; cc < BW_CPW; ++cc) {
+                double a = 0.0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) a = fma((double)l[cc][k], xv[k], a);
+                buf[wave][cc][lane] = a;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int col = lane >> 3, q = lane & 7;
+            double v = 0.0;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) v += buf[wave][col][8 * q + t];
+            v += __shfl_xor(v, 1);
+            v += __shfl_xor(v, 2);
+            v += __shfl_xor(v, 4);
+            if (q == 0 && c0 + col < bI)
+                part[((int64_t)sp * nrhs + j0 + j) * bI + c0 + col] = v;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();                 // buf is rewritten next
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        }
     }
 }
 
 // Triangular GEMV on a diagonal big block, out[i] = sum_j M[i + j*ld] v[j]
 // over j <= i (lower: M = Winv, forward) or j >= i (upper: M = Winv^T,
-// backward), i, j < bI.  One wave per 64 x 64 tile (row block ib, column
-// chunk jb) of the triangle; lane = row; the tile's 64 column loads are all in
-// flight at once.  Partial sums go to part[(jb * nrhs + j) * bI + i];
-// tri_gemv_reduce adds the chunks of a row in order (deterministic).
-constexpr int TG = 64;
+// backward), i, j < bI, where v[j] = V[j] - sum_{s < S} pin[(s nrhs + r) bI + j]
+// (the off-diagonal GEMV's partial sums, added in split order).  One wave per
+// 64 x 64 tile (row block ib, column chunk jb) of the triangle; lane = row;
+// the tile's 64 column loads are all in flight at once.  Partial sums go to
+// part[(jb * nrhs + j) * bI + i]; tri_gemv_reduce adds the chunks of a row
+// in order (deterministic).
 template <bool UPPER>
 __global__ __launch_bounds__(64) void tri_gemv_tile_kernel(const float* __restrict__ M, int ld,
                                                           int bI, const double* __restrict__ V,
                                                           int64_t ldv, int nrhs,
+                                                          const double* __restrict__ pin, int S,
                                                           double* __restrict__ part) {
     // tile t -> (ib, jb): lower ib >= jb, t = ib (ib + 1) / 2 + jb; upper mirrors it
     const int t = blockIdx.x;
@@ -267,20 +363,29 @@ __global__ __launch_bounds__(64) void tri_gemv_tile_kernel(const float* __restri
             m[u] = keep ? m[u] : 0.f;
         }
     }
+    // lane u builds v[j0 + u]; the products read it back by lane index
+    const int jl = j0 + lane < bI ? j0 + lane : bI - 1;
     for (int r = 0; r < nj; ++r) {
-        const double* v = V + (int64_t)(r0 + r) * ldv + j0;
-        double acc = 0.0;
-        if (full) {
+        double sum = 0.0;
+        const double* pr = pin + (int64_t)(r0 + r) * bI + jl;
+        const int64_t sstride = (int64_t)nrhs * bI;
+        int sp = 0;
+        for (; sp + 8 <= S; sp += 8) {       // batches of 8 independent loads
+            double pv[8];
 #pragma unroll
-            for (int u = 0; u < TG; ++u) acc = fma((double)m[u], v[u], acc);
-        } else {
+            for (int q = 0; q < 8; ++q) pv[q] = pr[(sp + q) * sstride];
 #pragma unroll
-            for (int u = 0; u < TG; ++u) {
-                const double vu = v[j0 + u < bI ? u : bI - 1 - j0];
-                acc = fma((double)m[u], vu, acc);
-            }
+            for (int q = 0; q < 8; ++q) sum += pv[q];
         }
-        if (i < bI) part[((int64_t)jb * nrhs + r0 + r) * bI + i] = acc;
+        for (; sp < S; ++sp) sum += pr[sp * sstride];
+        const double vv = V[(int64_t)(r0 + r) * ldv + jl] - sum;
+        double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+        for (int u = 0; u < TG; u += 2) {
+            acc0 = fma((double)m[u], rdlane_d(vv, u), acc0);
+            acc1 = fma((double)m[u + 1], rdlane_d(vv, u + 1), acc1);
+        }
+        if (i < bI) part[((int64_t)jb * nrhs + r0 + r) * bI + i] = acc0 + acc1;
     }
 }
 
@@ -309,46 +414,14 @@ __global__ __launch_bounds__(256) void tri_gemv_reduce_kernel(const double* __re
     out[i + (int64_t)r * ldo] = s;
 }
 
-// W[c][j] -= sum_{r < bI} L[p0 + r][c] * X[p0 + r][j]   for c < p0
-// one wave per column c (a contiguous run of bI floats): lane r holds rows
-// r, r + 64, ... (all loads in flight), then a wave reduction per rhs.
-constexpr int BU_MAXL = 32;     // bI <= 64 * BU_MAXL = 2048
-__global__ __launch_bounds__(256) void bwd_update_kernel(const float* __restrict__ L, int64_t ldl,
-                                                        int64_t p0, int bI,
-                                                        const double* __restrict__ X, int64_t ldx,
-                                                        double* __restrict__ W, int64_t ldw,
-                                                        int nrhs) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t c = (int64_t)blockIdx.x * 4 + wave;
-    if (c >= p0) return;
-    const int j0 = blockIdx.y * NR;
-    const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
-    const float* col = L + p0 + c * ldl;
-    float l[BU_MAXL];
-#pragma unroll
-    for (int u = 0; u < BU_MAXL; ++u) {      // clamped loads, then select (no branches)
-        const int rr = lane + 64 * u;
-        const float x = col[rr < bI ? rr : bI - 1];
-        l[u] = rr < bI ? x : 0.f;
-    }
-    for (int j = 0; j < nj; ++j) {
-        const double* x = X + p0 + (int64_t)(j0 + j) * ldx;
-        double acc = 0.0;
-#pragma unroll
-        for (int u = 0; u < BU_MAXL; ++u) {      // l[u] = 0 beyond bI: clamped load, no branch
-            const int rr = lane + 64 * u;
-            acc = fma((double)l[u], x[rr < bI ? rr : bI - 1], acc);
-        }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o);
-        if (lane == 0) W[c + (int64_t)(j0 + j) * ldw] -= acc;
-    }
-}
-
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
+// 128x128 tiles when they give at least two workgroups per CU of a 256-CU
+// device, else 64x64 tiles (the deep levels of the doubling are a few large
+// products: four times the workgroups for the same work)
 int launch_gemm(hipStream_t st, const gemm_batch& g, int nbatch) {
-    if (g.m >= 128 && g.n >= 128) {
+    const int64_t t128 = (int64_t)cdiv(g.m, 128) * cdiv(g.n, 128) * nbatch;
+    if (g.m >= 128 && g.n >= 128 && t128 >= 512) {
         dim3 grid(cdiv(g.m, 128), cdiv(g.n, 128), (unsigned)nbatch);
         hipLaunchKernelGGL(gemm_f32_kernel<128>, grid, dim3(256), 0, st, g);
     } else {
@@ -445,55 +518,75 @@ int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_
     return mpc_build_block_inverses_range(st, L, ldl, n, Bo, dinv, W, WT, 0, (n + Bo - 1) / Bo);
 }
 
-// z = L^{-T} L^{-1} w for nrhs columns.  W (in/out, destroyed), Y (work), Z (out)
 static int tri_gemv(hipStream_t st, const float* M, int Bo, int bI, int upper, const double* V,
-                    int64_t ldv, int nrhs, double* part, double* out, int64_t ldo) {
+                    int64_t ldv, int nrhs, const double* pin, int S, double* part, double* out,
+                    int64_t ldo) {
     const int nb = (bI + TG - 1) / TG;
     const dim3 grid((unsigned)(nb * (nb + 1) / 2), cdiv(nrhs, NR));
     if (upper)
         hipLaunchKernelGGL(tri_gemv_tile_kernel<true>, grid, dim3(64), 0, st, M, Bo, bI, V, ldv,
-                           nrhs, part);
+                           nrhs, pin, S, part);
     else
         hipLaunchKernelGGL(tri_gemv_tile_kernel<false>, grid, dim3(64), 0, st, M, Bo, bI, V, ldv,
-                           nrhs, part);
+                           nrhs, pin, S, part);
     hipLaunchKernelGGL(tri_gemv_reduce_kernel, dim3(cdiv(bI, 256), (unsigned)nrhs), dim3(256), 0, st,
                        part, bI, nrhs, upper, out, ldo);
     MPC_LAUNCH_CHECK();
     return 0;
 }
 
-int64_t mpc_block_prec_partial_doubles(int Bo, int nrhs) {
-    return (int64_t)((Bo + TG - 1) / TG) * Bo * nrhs;
+// splits of the off-diagonal GEMVs: at most n / FW_CW (forward, over p0 < n
+// columns) and n / BW_RW (backward, over fewer than n rows)
+static int64_t gemv_splits_max(int64_t n, int Bo) {
+    (void)Bo;
+    return std::max<int64_t>((n + FW_CW - 1) / FW_CW, (n + BW_RW - 1) / BW_RW);
 }
 
+int64_t mpc_block_prec_partial_doubles(int64_t n, int Bo, int nrhs) {
+    return ((int64_t)(Bo + TG - 1) / TG + gemv_splits_max(n, Bo)) * Bo * nrhs;
+}
+
+// z = L^{-T} L^{-1} r for nrhs columns: R (in), Y (forward result), Z (out).
+// late_ev (optional): the stream waits for it before the first use of big
+// block I_late's inverse (which may still be in the making).
 int mpc_apply_block_prec(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
-                         const float* Winv, const float* WinvT, double* W, double* Y, double* Z,
-                         int64_t ld, int nrhs, double* part) {
-    if (Bo > 64 * BU_MAXL) {
-        mpc_set_error("apply_block_prec: big block %d exceeds %d", Bo, 64 * BU_MAXL);
-        return MPC_ERR_ARG;
-    }
+                         const float* Winv, const float* WinvT, const double* R, double* Y,
+                         double* Z, int64_t ld, int nrhs, double* part, hipEvent_t late_ev,
+                         int64_t I_late) {
     const int64_t nI = (n + Bo - 1) / Bo;
     const unsigned gz = cdiv(nrhs, NR);
+    double* gpart = part + (int64_t)((Bo + TG - 1) / TG) * Bo * nrhs;   // GEMV partial sums
     int rc;
     for (int64_t I = 0; I < nI; ++I) {
         const int64_t p0 = I * Bo;
         const int bI = (int)(n - p0 < Bo ? n - p0 : Bo);
+        int S = 0;
+        if (p0 > 0) {
+            S = (int)cdiv(p0, FW_CW);
+            hipLaunchKernelGGL(fwd_gemv_partial_kernel, dim3(cdiv(bI, 64), (unsigned)S, gz),
+                               dim3(256), 0, st, L, ldl, p0, bI, Y, ld, nrhs, gpart);
+        }
         const float* Wi = Winv + I * (int64_t)Bo * Bo;
-        if ((rc = tri_gemv(st, Wi, Bo, bI, 0, W + p0, ld, nrhs, part, Y + p0, ld))) return rc;
-        const int64_t r0 = p0 + bI;
-        if (r0 < n)
-            hipLaunchKernelGGL(fwd_update_kernel, dim3(cdiv(n - r0, 64), gz), dim3(64 * FU_WAVES), 0,
-                               st, L, ldl, n, r0, p0, bI, Y, ld, W, ld, nrhs);
+        if (late_ev && I == I_late && hipStreamWaitEvent(st, late_ev, 0) != hipSuccess) {
+            mpc_set_error("apply_block_prec: stream wait failed");
+            return MPC_ERR_HIP;
+        }
+        if ((rc = tri_gemv(st, Wi, Bo, bI, 0, R + p0, ld, nrhs, gpart, S, part, Y + p0, ld)))
+            return rc;
     }
     for (int64_t I = nI - 1; I >= 0; --I) {
         const int64_t p0 = I * Bo;
         const int bI = (int)(n - p0 < Bo ? n - p0 : Bo);
+        const int64_t p1 = p0 + bI;
+        int S = 0;
+        if (p1 < n) {
+            S = (int)cdiv(n - p1, BW_RW);
+            hipLaunchKernelGGL(bwd_gemv_partial_kernel, dim3(cdiv(bI, 4 * BW_CPW), (unsigned)S, gz),
+                               dim3(256), 0, st, L, ldl, p0, bI, n, Z, ld, nrhs, gpart);
+        }
         const float* WTi = WinvT + I * (int64_t)Bo * Bo;
-        if ((rc = tri_gemv(st, WTi, Bo, bI, 1, Y + p0, ld, nrhs, part, Z + p0, ld))) return rc;
-        if (p0 > 0)
-            hipLaunchKernelGGL(bwd_update_kernel, dim3(cdiv(p0, 4), gz), dim3(256), 0, st, L, ldl,
-                               p0, bI, Z, ld, Y, ld, nrhs);
+        if ((rc = tri_gemv(st, WTi, Bo, bI, 1, Y + p0, ld, nrhs, gpart, S, part, Z + p0, ld)))
+            return rc;
     }
     MPC_LAUNCH_CHECK();
     return 0;

@@ -74,6 +74,11 @@ struct mpc_handle {
     hipStream_t  wstream = nullptr;
     hipEvent_t   ev_wready = nullptr;  // a big block of L is final
     hipEvent_t   ev_wdone = nullptr;   // all inverses built (on wstream)
+    hipEvent_t   ev_wpre = nullptr;    // all but the last big block's built (on wstream)
+    // the factor returns with the inverses still being built on wstream:
+    // users of winv / dinv / the factor buffer wait for ev_wdone first (the
+    // first preconditioner apply of a PCG only before its last big block)
+    bool         winv_pending = false;
     // factor workspace
     mpc_workspace panel;      // quantised panel payload, n * nb * 4 bytes
     mpc_workspace scales;     // n * nb/32 bytes
@@ -207,10 +212,12 @@ int mpc_build_block_inverses_range(hipStream_t st, const float* L, int64_t ldl, 
 int mpc_build_block_inverses(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
                              const float* dinv, float* W, float* WT);
 // part: mpc_block_prec_partial_doubles(Bo, nrhs) doubles of scratch
+// z = L^-T L^-1 r (R in, Y work, Z out; ld each)
 int mpc_apply_block_prec(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
-                         const float* Winv, const float* WinvT, double* W, double* Y, double* Z,
-                         int64_t ld, int nrhs, double* part);
-int64_t mpc_block_prec_partial_doubles(int Bo, int nrhs);
+                         const float* Winv, const float* WinvT, const double* R, double* Y,
+                         double* Z, int64_t ld, int nrhs, double* part,
+                         hipEvent_t late_ev = nullptr, int64_t I_late = -1);
+int64_t mpc_block_prec_partial_doubles(int64_t n, int Bo, int nrhs);
 inline int mpc_block_bo(int64_t n) { return n > 8192 ? 2048 : (n > 2048 ? 1024 : 256); }
 
 // host-side generators shared with mpc_matgen (RNG identical to the device one)
