@@ -551,31 +551,34 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         // of block column j+1 by panel j (one mixed-class launch of small
         // tiles), then POTRF, TRSM and quantisation of panel j+1.  st
         // carries the rest of panel j's update (block columns beyond j+1),
-        // which overlaps that chain.  Ordering: the bulk update of j waits
-        // for panel j; the block-column update of j waits for the bulk
-        // update of j-1 (the last other writer of block column j+1).  The
-        // panel buffers of parity j&1 are rewritten by panel j+2, which
-        // pstream runs after the block-column update of j+1, itself after
-        // the bulk update of j.
+        // which overlaps the panel.  Ordering: the bulk update of j waits
+        // for the block-column update of j (and so for panel j); the
+        // block-column update of j waits for the bulk update of j-1 (the
+        // last other writer of block column j+1).  The panel buffers of
+        // parity j&1 are rewritten by panel j+2, which pstream runs after
+        // the block-column update of j+1, itself after the bulk update of j.
         hipStream_t ps = h->pstream;
         MPC_HIP(hipEventRecord(h->ev_col, st));
         MPC_HIP(hipStreamWaitEvent(ps, h->ev_col, 0));
         MPC_HIP(hipStreamWaitEvent(h->wstream, h->ev_col, 0));
         if ((rc = panel(0, ps))) return rc;
         if ((rc = build_w(0, ps))) return rc;
-        MPC_HIP(hipEventRecord(h->ev_panel, ps));
         for (int64_t j = 0; j + 1 < nt; ++j) {
             const int64_t bs1 = std::min<int64_t>(nb, n - (j + 1) * nb);
             const int64_t split = (bs1 + MPC_SYRK_TILE - 1) / MPC_SYRK_TILE;
-            MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));              // panel j
             if (j > 0) MPC_HIP(hipStreamWaitEvent(ps, h->ev_bulk, 0));    // bulk j-1
+            if ((rc = update(j, 0, 2 * split, ps, 1))) return rc;
+            // the bulk starts once the block column is done: the column's
+            // small launch then has the chip to itself (sharing it with the
+            // bulk's workgroups made it several times slower)
+            MPC_HIP(hipEventRecord(h->ev_panel, ps));
+            MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
             if ((rc = update(j, split, -1, st, o->update_mixed ? 2 : 0))) return rc;
             MPC_HIP(hipEventRecord(h->ev_bulk, st));
-            if ((rc = update(j, 0, 2 * split, ps, 1))) return rc;
             if ((rc = panel(j + 1, ps))) return rc;
             if ((rc = build_w(j + 1, ps))) return rc;
-            MPC_HIP(hipEventRecord(h->ev_panel, ps));
         }
+        MPC_HIP(hipEventRecord(h->ev_panel, ps));
         MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
     } else {
         // sequential with per-phase events

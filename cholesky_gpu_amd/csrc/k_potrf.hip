@@ -81,6 +81,58 @@ __device__ __forceinline__ void update_subtile(float* T, int ib, int jb, int kb)
     for (int i = 0; i < 16; ++i) Cs[sig(i, h) + n * SUBLD] = acc[i];
 }
 
+// Columns [c_lo, c_hi) of the lower block into the sub-tiles (identity
+// beyond bs), 16-byte path: wave w of nwav reads columns c_lo + w, +nwav, ...
+// (LB of them per batch, all loads of a batch in flight), lane l rows
+// 4l..4l+3.  Needs a 16-byte aligned block with lda and bs multiples of 4;
+// rows above the diagonal are read and dropped.
+template <int LB>
+__device__ __forceinline__ void load_cols_vec(float* T, const float* __restrict__ A, int64_t lda,
+                                              int bs, int nsb, int c_lo, int c_hi, int w,
+                                              int nwav) {
+    const int lane = threadIdx.x & 63;
+    const int r4 = 4 * lane, rb = r4 >> 5;
+    for (int cbase = c_lo; cbase < c_hi; cbase += nwav * LB) {
+        float4 v[LB];
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+            // clamped, unconditional load (selected below): a load under a
+            // per-lane condition becomes a branch with its own wait
+            const int c = cbase + nwav * u + w;
+            const int cc = c < bs ? c : bs - 1, rc = r4 < bs ? r4 : bs - 4;
+            const float4 x = *reinterpret_cast<const float4*>(A + rc + (int64_t)cc * lda);
+            v[u] = (c < bs && r4 < bs) ? x : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+            const int c = cbase + nwav * u + w, cb = c >> 5;
+            if (c >= c_hi || rb >= nsb || rb < cb) continue;
+            const float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int r = r4 + q;
+                float x = e[q];
+                if (r < c) x = 0.f;
+                if (r >= bs || c >= bs) x = (r == c) ? 1.f : 0.f;
+                T[sidx(rb, cb) * SUBSZ + (r & 31) + (c & 31) * SUBLD] = x;
+            }
+        }
+    }
+}
+
+// Sub-column cb (its lower part, rows cb*32 .. bs-1) back to A by threads
+// t0 .. t0+nthr-1: a thread per row, consecutive threads consecutive rows.
+__device__ __forceinline__ void store_subcol(const float* T, float* __restrict__ A, int64_t lda,
+                                             int bs, int cb, int t0, int nthr) {
+    const int c0 = cb * 32, c1 = bs < c0 + 32 ? bs : c0 + 32;
+    for (int r = c0 + (int)threadIdx.x - t0; r < bs; r += nthr) {
+        const int cend = r + 1 < c1 ? r + 1 : c1;        // lower triangle only
+        const float* Ts = T + sidx(r >> 5, cb) * SUBSZ + (r & 31);
+        float* Ar = A + r;
+        for (int c = c0; c < cend; ++c) Ar[(int64_t)c * lda] = Ts[(c & 31) * SUBLD];
+    }
+}
+
 __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_t lda, int bs,
                                                     int col0, float* __restrict__ dinv,
                                                     mpc_small* __restrict__ sm, int ntiles_next,
@@ -103,41 +155,11 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
     const int nsb = (bs + 31) >> 5;
 
     // ---- load lower sub-tiles (padding: identity beyond bs), with many loads
-    // per thread in flight (one batch per HBM latency).  16-byte path: lane l
-    // of wave w reads rows 4l..4l+3 of columns w, w+NW, ...; it needs a 16-byte
-    // aligned block with lda and bs multiples of 4 (reads stay inside the
-    // block's columns; rows above the diagonal are read and dropped).
+    // per thread in flight.  16-byte path: sub-column 0 now; the other
+    // sub-columns are read by waves 1.. while wave 0 factors sub-block 0.
     const bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && (lda % 4 == 0) && (bs % 4 == 0);
     if (vec) {
-        constexpr int LB = 32;
-        const int r4 = 4 * lane, rb = r4 >> 5;
-        const int ncol = nsb * 32;
-        for (int cbase = 0; cbase < ncol; cbase += NW * LB) {
-            float4 v[LB];
-#pragma unroll
-            for (int u = 0; u < LB; ++u) {
-                // clamped, unconditional load (selected below): a load under a
-                // per-lane condition becomes a branch with its own wait
-                const int c = cbase + NW * u + wave;
-                const int cc = c < bs ? c : bs - 1, rc = r4 < bs ? r4 : bs - 4;
-                const float4 x = *reinterpret_cast<const float4*>(A + rc + (int64_t)cc * lda);
-                v[u] = (c < bs && r4 < bs) ? x : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-#pragma unroll
-            for (int u = 0; u < LB; ++u) {
-                const int c = cbase + NW * u + wave, cb = c >> 5;
-                if (c >= ncol || rb >= nsb || rb < cb) continue;
-                const float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int r = r4 + q;
-                    float x = e[q];
-                    if (r < c) x = 0.f;
-                    if (r >= bs || c >= bs) x = (r == c) ? 1.f : 0.f;
-                    T[sidx(rb, cb) * SUBSZ + (r & 31) + (c & 31) * SUBLD] = x;
-                }
-            }
-        }
+        load_cols_vec<8>(T, A, lda, bs, nsb, 0, 32, wave, NW);
     } else {
         constexpr int LB = 32;
         const int r = tid & 255, rb = r >> 5, half = tid >> 8;
@@ -165,6 +187,8 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
         // ---- 1. factor + invert the diagonal sub-tile (wave 0); meanwhile
         // waves 1..3 finish the trailing update of kb-1 on the sub-columns
         // beyond kb (3b below), which the factor of sub-block kb does not read
+        if (wave != 0 && kb == 0 && vec)                 // the rest of the block
+            load_cols_vec<16>(T, A, lda, bs, nsb, 32, nsb * 32, wave - 1, NW - 1);
         if (wave != 0 && kb >= 1) {
             const int m = nsb - 1 - kb;                  // sub-columns kb+1 .. nsb-1
             const int npair = m * (m + 1) / 2;
@@ -174,6 +198,8 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                 const int jj = pi - ii * (ii + 1) / 2;
                 update_subtile(T, kb + 1 + ii, kb + 1 + jj, kb - 1);
             }
+            // sub-column kb-1 is final: back to A while the factor runs
+            store_subcol(T, A, lda, bs, kb - 1, 64, NT - 64);
         }
         if (wave == 0) {
             // Pipelined right-looking factor, row per lane (lanes 32..63 mirror
@@ -203,9 +229,14 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                     if (bad < 0) bad = k;
                     piv = 1.f;
                 }
-                const float l = sqrtf(piv);
-                float rl = __builtin_amdgcn_rcpf(l);  // one Newton step: rounded reciprocal
-                rl = fmaf(fmaf(-l, rl, 1.f), rl, rl);
+                // 1/sqrt(piv) from the hardware estimate and one Newton step
+                // (about 1 ulp), sqrt(piv) = piv / sqrt(piv): a short
+                // dependent chain per pivot, where a correctly rounded sqrt
+                // and reciprocal take four times the instructions
+                float rl = __builtin_amdgcn_rsqf(piv);
+                const float hr = 0.5f * piv * rl;
+                rl = fmaf(fmaf(-hr, rl, 0.5f), rl, rl);
+                const float l = piv * rl;
                 d[k] = (r == k) ? l : d[k] * rl;
                 if (r == k) Lr[32 * LRLD + k] = rl;
                 if (k < 31) {
@@ -315,19 +346,13 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
         mpc_lds_barrier();
     }
 
-    // ---- write back the lower triangle (row r, every (NT/256)-th column per
-    // thread), min pivot^2, failure
+    // ---- write back the last sub-column (the others went back during the
+    // factor), min pivot^2, failure
+    store_subcol(T, A, lda, bs, nsb - 1, 0, NT);
     float pmin = INFINITY;
-    {
-        const int r = tid & 255, rb = r >> 5, half = tid >> 8;
-        if (r < bs) {
-            for (int c = half; c <= r; c += NT / 256) {
-                const float v = T[sidx(rb, c >> 5) * SUBSZ + (r & 31) + (c & 31) * SUBLD];
-                A[r + (int64_t)c * lda] = v;
-            }
-            const float d = T[sidx(rb, rb) * SUBSZ + (r & 31) * (1 + SUBLD)];
-            pmin = d * d;
-        }
+    if (tid < bs) {
+        const float d = T[sidx(tid >> 5, tid >> 5) * SUBSZ + (tid & 31) * (1 + SUBLD)];
+        pmin = d * d;
     }
     if (tid < 256) red[tid] = pmin;
     mpc_lds_barrier();
