@@ -36,8 +36,8 @@ def main():
     ap.add_argument("--n", type=int, default=16384)
     ap.add_argument("--nb", type=int, default=256)
     ap.add_argument("--eps", type=float, default=0.125)
-    ap.add_argument("--update-mixed", type=int, default=0,
-                    help="1: bulk trailing update as one launch with all precision classes")
+    ap.add_argument("--agg", type=int, default=None,
+                    help="panels per aggregated trailing update (library default 2)")
     ap.add_argument("--ladder", default="e4m3,fp32",
                     help="per-tile precision ladder of the mixed run, lowest first")
     ap.add_argument("--syrk-reps", type=int, default=20)
@@ -77,9 +77,9 @@ def main():
              "mxe5m2": L.PREC_MXE5M2, "bf16": L.PREC_BF16, "fp16": L.PREC_FP16, "fp32": L.PREC_FP32}
     ladder = tuple(names[x] for x in args.ladder.split(","))
     fo8 = S.factor_opts(nb=nb, update_prec=L.PREC_SWITCH, ladder=ladder, eps_prime=args.eps,
-                        fallback=1, update_mixed=args.update_mixed)
+                        fallback=1, agg_panels=args.agg)
     fo32 = S.factor_opts(nb=nb, update_prec=L.PREC_FP32, ladder=(L.PREC_FP32,), fallback=0,
-                         update_mixed=args.update_mixed)
+                         agg_panels=args.agg)
     po = S.pcg_opts(tol=1e-10, maxit=500)
 
     def solve(fo):

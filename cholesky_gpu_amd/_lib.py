@@ -36,7 +36,7 @@ class MatgenOpts(C.Structure):
 class FactorOpts(C.Structure):
     _fields_ = [("nb", C.c_int), ("update_prec", C.c_int), ("ladder", C.c_int * 4),
                 ("n_ladder", C.c_int), ("eps_prime", C.c_double), ("fallback", C.c_int),
-                ("profile", C.c_int), ("update_mixed", C.c_int)]
+                ("profile", C.c_int), ("agg_panels", C.c_int)]
 
 
 class FactorInfo(C.Structure):

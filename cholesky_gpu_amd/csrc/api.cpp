@@ -59,12 +59,13 @@ __global__ void init_small_kernel(mpc_small* sm) {
         sm->nan_flag = 0;
         sm->mu_bits = 0x7f800000u;
     }
-    for (int t = threadIdx.x; t < 1024; t += blockDim.x)
+    for (int t = threadIdx.x; t < 1024; t += blockDim.x) {
         for (int b = 0; b < 2; ++b) {
             sm->tile_amax[b][t] = 0u;
             sm->tile_maxrs[b][t] = 0u;
-            sm->tile_prec[b][t] = 0;
         }
+        for (int b = 0; b < MPC_PANEL_SLOTS; ++b) sm->tile_prec[b][t] = 0;
+    }
 }
 __global__ void init_factor_kernel(const float* A, int64_t lda, int64_t n, float* tracker,
                                    uint8_t* map, int64_t nt) {
@@ -111,12 +112,14 @@ extern "C" int mpc_handle_create(mpc_handle** out, int device) {
     h->own_stream = true;
     e = hipStreamCreateWithFlags(&h->pstream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->wstream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&h->fstream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_col, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_panel, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wready, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wdone, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_wpre, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_bulk, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&h->ev_far, hipEventDisableTiming);
     int rc = 0;
     if (e != hipSuccess) {
         mpc_set_error("stream/event creation failed: %s", hipGetErrorString(e));
@@ -149,6 +152,7 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->pstream) hipStreamSynchronize(h->pstream);
     if (h->wstream) hipStreamSynchronize(h->wstream);
+    if (h->fstream) hipStreamSynchronize(h->fstream);
     for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
                              &h->pcg, &h->scratch, &h->winv, &h->gp64, &h->gp32, &h->gpvec})
         ws_free(*w);
@@ -158,8 +162,10 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     if (h->ev_wdone) hipEventDestroy(h->ev_wdone);
     if (h->ev_wpre) hipEventDestroy(h->ev_wpre);
     if (h->ev_bulk) hipEventDestroy(h->ev_bulk);
+    if (h->ev_far) hipEventDestroy(h->ev_far);
     if (h->pstream) hipStreamDestroy(h->pstream);
     if (h->wstream) hipStreamDestroy(h->wstream);
+    if (h->fstream) hipStreamDestroy(h->fstream);
     if (h->own_stream) hipStreamDestroy(h->stream);
     delete h;
     return 0;
@@ -415,12 +421,13 @@ extern "C" void mpc_factor_opts_default(mpc_factor_opts* o) {
     o->eps_prime = 0.25;
     o->fallback = 1;
     o->profile = 0;
-    o->update_mixed = 0;
+    o->agg_panels = 2;
 }
 
 static int validate_factor_opts(const mpc_factor_opts* o) {
     MPC_CHECK(o->nb == 64 || o->nb == 128 || o->nb == 256, "factor: nb must be 64, 128 or 256");
-    MPC_CHECK(o->update_mixed == 0 || o->update_mixed == 1, "factor: update_mixed must be 0 or 1");
+    MPC_CHECK(o->agg_panels >= 1 && o->agg_panels <= MPC_AGG_MAX, "factor: agg_panels must be 1..%d",
+              MPC_AGG_MAX);
     if (o->update_prec == MPC_PREC_SWITCH) {
         MPC_CHECK(o->n_ladder >= 1 && o->n_ladder <= 4, "factor: n_ladder must be 1..4");
         bool f16 = false, other16 = false;
@@ -474,8 +481,15 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     hipStream_t st = h->stream;
     int rc;
     if ((rc = wait_winv(h, st))) return rc;       // it reads dinv and the factor
-    if ((rc = mpc_ws_reserve(h->panel, 2 * (size_t)(n + 128) * nb * 4))) return rc;
-    if ((rc = mpc_ws_reserve(h->scales, 2 * (size_t)(n + 128) * (nb / 32)))) return rc;
+    // panels live in a ring: 2 buffers, or 2 * agg_panels when the updates
+    // beyond a group of agg_panels block columns are aggregated (the group
+    // being aggregated and the group being factored); aggregation needs
+    // precision tiles of exactly one block (tile_rows == nb)
+    const bool prof = o->profile != 0;
+    const int AG = (prof || tr != nb) ? 1 : o->agg_panels;
+    const int nslot = 2 * AG;
+    if ((rc = mpc_ws_reserve(h->panel, (size_t)nslot * (n + 128) * nb * 4))) return rc;
+    if ((rc = mpc_ws_reserve(h->scales, (size_t)nslot * (n + 128) * (nb / 32)))) return rc;
     if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nt * nb / 32 + 8) * 1024 * sizeof(float)))) return rc;
     if ((rc = mpc_ws_reserve(h->rowstat, (size_t)(n + 128) * sizeof(float)))) return rc;
     mpc_small* sm = (mpc_small*)h->small.ptr;
@@ -495,10 +509,11 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
 
     const int64_t pstride = (int64_t)(n + 128) * nb * 4;     // payload bytes per buffer
     const int64_t sstride = (int64_t)(n + 128) * (nb / 32);  // scale bytes per buffer
-    // POTRF, TRSM and quantisation of panel j on stream `ps`, into the step-parity buffers
+    // POTRF, TRSM and quantisation of panel j on stream `ps`: tile statistics
+    // in the step-parity buffers, the quantised panel in ring slot j % nslot
     auto panel = [&](int64_t j, hipStream_t ps) -> int {
         const int64_t c0 = j * nb, bs = std::min<int64_t>(nb, n - c0), e = c0 + bs, s = n - e;
-        const int ntl = (int)((s + tr - 1) / tr), b = (int)(j & 1);
+        const int ntl = (int)((s + tr - 1) / tr), b = (int)(j & 1), q = (int)(j % nslot);
         float* Ajj = dA + c0 + c0 * lda;
         int r;
         if ((r = mpc_launch_potrf(ps, Ajj, lda, (int)bs, (int)c0, dinv, sm, ntl,
@@ -512,18 +527,39 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
             return r;
         return mpc_launch_quantize_ex(ps, A21, s, nb, lda, tr, nullptr, sm->tile_amax[b],
                                       sm->tile_maxrs[b], sm, o->update_prec, ladder, o->n_ladder,
-                                      eps, sm->tile_prec[b], map ? map + (j + 1) + j * nt : nullptr,
-                                      payload + b * pstride, (int64_t)nb * 4, scales + b * sstride,
+                                      eps, sm->tile_prec[q], map ? map + (j + 1) + j * nt : nullptr,
+                                      payload + q * pstride, (int64_t)nb * 4, scales + q * sstride,
                                       &sm->nan_flag);
     };
     // trailing update of panel j on stream us restricted to SYRK tile columns
     // [t0, t1) of A22 (tiles of MPC_SYRK_TILE, or of 64 * mixed when mixed)
     auto update = [&](int64_t j, int64_t t0, int64_t t1, hipStream_t us, int mixed) -> int {
         const int64_t e = std::min<int64_t>((j + 1) * nb, n), s = n - e;
-        const int b = (int)(j & 1);
-        return mpc_launch_syrk_classes(us, dA + e + e * lda, s, lda, nb, payload + b * pstride,
-                                       (int64_t)nb * 4, scales + b * sstride, tr,
-                                       sm->tile_prec[b], sm, cmask, fmask, t0, t1, mixed);
+        const int q = (int)(j % nslot);
+        return mpc_launch_syrk_classes(us, dA + e + e * lda, s, lda, nb, payload + q * pstride,
+                                       (int64_t)nb * 4, scales + q * sstride, tr,
+                                       sm->tile_prec[q], sm, cmask, fmask, t0, t1, mixed);
+    };
+    // aggregated update of panels p0 .. p0+np-1 on the trailing matrix from
+    // block column E (rows and columns >= E nb), tile columns [t0, t1) of
+    // 64 * qa; qa = 1 one mixed-class launch, qa = 2 one launch per class
+    auto agg = [&](int64_t p0, int np, int64_t E, int64_t t0, int64_t t1, hipStream_t us,
+                   int qa) -> int {
+        const int64_t e = E * nb;
+        if (e >= n || np < 1) return 0;
+        mpc_syrk_panels P{};
+        P.payload = payload;
+        P.scales = scales;
+        P.prec = &sm->tile_prec[0][0];
+        P.pstride = pstride;
+        P.sstride = sstride;
+        P.rows0 = e - (p0 + 1) * nb;
+        P.slots = nslot;
+        P.p0 = (int)(p0 % nslot);
+        P.np = np;
+        P.nb = nb;
+        return mpc_launch_syrk_panels(us, dA + e + e * lda, n - e, lda, nb, P, (int64_t)nb * 4, tr,
+                                      sm, t0, t1, qa, cmask, fmask);
     };
 
     const int Bo = mpc_block_bo(n);
@@ -540,13 +576,66 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         return mpc_build_block_inverses_range(h->wstream, dA, lda, n, Bo, dinv, W, WT, I, I + 1);
     };
 
-    const bool prof = o->profile != 0;
     double tp = 0, tt = 0, tq = 0, ts = 0;
     hipEvent_t e0, e1;
     MPC_HIP(hipEventCreate(&e0));
     MPC_HIP(hipEventCreate(&e1));
     MPC_HIP(hipEventRecord(e0, st));
-    if (!prof) {
+    if (!prof && AG > 1) {
+        // One-deep look-ahead with aggregated far updates.  Block columns
+        // come in groups of AG.  Inside a group, as below: panel j updates
+        // block column j+1 on pstream and the rest of the group's columns on
+        // st.  The columns beyond the group are updated once per group with
+        // all AG panels (K = AG nb): block column E (the next group's first)
+        // on pstream, the rest of the next group on st, everything beyond
+        // that on fstream, where it overlaps the next group's panels.
+        // Ordering: a writer of a block column waits for its previous writer
+        // (ev_bulk: st's last update; ev_far: fstream's last); a panel slot
+        // is reused AG panels after the group that read it was aggregated.
+        hipStream_t ps = h->pstream, fs = h->fstream;
+        MPC_HIP(hipEventRecord(h->ev_col, st));
+        MPC_HIP(hipStreamWaitEvent(ps, h->ev_col, 0));
+        MPC_HIP(hipStreamWaitEvent(fs, h->ev_col, 0));
+        MPC_HIP(hipStreamWaitEvent(h->wstream, h->ev_col, 0));
+        if ((rc = panel(0, ps))) return rc;
+        if ((rc = build_w(0, ps))) return rc;
+        bool far_pending = false;
+        for (int64_t j = 0; j + 1 < nt; ++j) {
+            const int64_t g = j / AG, E = std::min<int64_t>((g + 1) * AG, nt);
+            const int64_t bs1 = std::min<int64_t>(nb, n - (j + 1) * nb);
+            const int64_t split = (bs1 + MPC_SYRK_TILE - 1) / MPC_SYRK_TILE;
+            if (j > 0) MPC_HIP(hipStreamWaitEvent(ps, h->ev_bulk, 0));    // st's last update
+            if (j + 1 < E) {
+                // inside the group: block column j+1, then the group's other columns
+                if ((rc = update(j, 0, 2 * split, ps, 1))) return rc;
+                MPC_HIP(hipEventRecord(h->ev_panel, ps));
+                MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
+                const int64_t wend = (E - (j + 1)) * nb / MPC_SYRK_TILE;
+                if ((rc = update(j, split, wend, st, 0))) return rc;
+                MPC_HIP(hipEventRecord(h->ev_bulk, st));
+            } else {
+                // the group's last panel: aggregated update beyond the group
+                const int64_t p0 = g * AG;
+                const int np = (int)(j - p0 + 1);
+                if (far_pending) MPC_HIP(hipStreamWaitEvent(ps, h->ev_far, 0));
+                if ((rc = agg(p0, np, E, 0, nb / 64, ps, 1))) return rc;
+                MPC_HIP(hipEventRecord(h->ev_panel, ps));
+                MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
+                MPC_HIP(hipStreamWaitEvent(fs, h->ev_panel, 0));
+                if ((rc = agg(p0, np, E, nb / MPC_SYRK_TILE, (int64_t)AG * nb / MPC_SYRK_TILE, st, 2)))
+                    return rc;
+                MPC_HIP(hipEventRecord(h->ev_bulk, st));
+                if ((rc = agg(p0, np, E, (int64_t)AG * nb / MPC_SYRK_TILE, -1, fs, 2))) return rc;
+                MPC_HIP(hipEventRecord(h->ev_far, fs));
+                far_pending = true;
+            }
+            if ((rc = panel(j + 1, ps))) return rc;
+            if ((rc = build_w(j + 1, ps))) return rc;
+        }
+        MPC_HIP(hipEventRecord(h->ev_panel, ps));
+        MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
+        if (far_pending) MPC_HIP(hipStreamWaitEvent(st, h->ev_far, 0));
+    } else if (!prof) {
         // One-deep look-ahead.  pstream carries the critical chain: the update
         // of block column j+1 by panel j (one mixed-class launch of small
         // tiles), then POTRF, TRSM and quantisation of panel j+1.  st
@@ -573,7 +662,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
             // bulk's workgroups made it several times slower)
             MPC_HIP(hipEventRecord(h->ev_panel, ps));
             MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
-            if ((rc = update(j, split, -1, st, o->update_mixed ? 2 : 0))) return rc;
+            if ((rc = update(j, split, -1, st, 0))) return rc;
             MPC_HIP(hipEventRecord(h->ev_bulk, st));
             if ((rc = panel(j + 1, ps))) return rc;
             if ((rc = build_w(j + 1, ps))) return rc;

@@ -290,62 +290,126 @@ __device__ __forceinline__ void mm_16(const operand_src& oc, const operand_src& 
         }
     };
     // both operands stored in the class's own format: no per-load format branch
-    if (oc.prec == native && orow.prec == native)
+    if (KC > 0 && oc.prec == native && orow.prec == native)
         k_loop<KC, 16>(k, [&](int kc) { body(kc, std::true_type{}); });
     else
         k_loop<0, 16>(k, [&](int kc) { body(kc, std::false_type{}); });
 }
 
-template <int QA, int KC>
+// NATIVE (runtime-K form only): both operands stored in FP32
+template <int QA, int KC, bool NATIVE>
 __device__ __forceinline__ void mm_f32(const operand_src& oc, const operand_src& orow,
                                        const tile_ctx& t, int k, v16f acc[QA][QA]) {
     const int h = (threadIdx.x & 63) >> 5;
     int64_t ra[QA], rb[QA];
     operand_rows<QA>(t, ra, rb);
     // k permutation: half h takes the 8-element groups k = 16c + 8h .. +8
-    auto body = [&](int kc, auto both_native) {
-        const int k0 = kc + 8 * h;
-        float A[QA][8], B[QA][8];
+    if constexpr (KC > 0) {
+        // unrolled (small launches): branch-free loads for the all-FP32 pair
+        auto body = [&](int kc, auto both_native) {
+            const int k0 = kc + 8 * h;
+            float A[QA][8], B[QA][8];
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                if constexpr (decltype(both_native)::value) {
+                    const float* pa = reinterpret_cast<const float*>(oc.payload + ra[q] * oc.row_stride) + k0;
+                    const float* pb = reinterpret_cast<const float*>(orow.payload + rb[q] * orow.row_stride) + k0;
+                    const float4 a0 = *reinterpret_cast<const float4*>(pa), a1 = *reinterpret_cast<const float4*>(pa + 4);
+                    const float4 b0 = *reinterpret_cast<const float4*>(pb), b1 = *reinterpret_cast<const float4*>(pb + 4);
+                    A[q][0] = a0.x; A[q][1] = a0.y; A[q][2] = a0.z; A[q][3] = a0.w;
+                    A[q][4] = a1.x; A[q][5] = a1.y; A[q][6] = a1.z; A[q][7] = a1.w;
+                    B[q][0] = b0.x; B[q][1] = b0.y; B[q][2] = b0.z; B[q][3] = b0.w;
+                    B[q][4] = b1.x; B[q][5] = b1.y; B[q][6] = b1.z; B[q][7] = b1.w;
+                } else {
+                    load8_f32(oc, ra[q], k0, A[q]);
+                    load8_f32(orow, rb[q], k0, B[q]);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+#pragma unroll
+                for (int a = 0; a < QA; ++a)
+#pragma unroll
+                    for (int b = 0; b < QA; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[b][e], B[a][e], acc[a][b], 0, 0, 0);
+        };
+        if (oc.prec == MPC_PREC_FP32 && orow.prec == MPC_PREC_FP32)
+            k_loop<KC, 16>(k, [&](int kc) { body(kc, std::true_type{}); });
+        else
+            k_loop<KC, 16>(k, [&](int kc) { body(kc, std::false_type{}); });
+    } else if constexpr (NATIVE) {
+        // both operands FP32: one branch-free loop, the next chunk's operands
+        // loaded before this chunk's products (the FP32 class is MFMA-bound
+        // only if its loads overlap the products)
+        const float* pa[QA];
+        const float* pb[QA];
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
-            if constexpr (decltype(both_native)::value) {
-                const float* pa = reinterpret_cast<const float*>(oc.payload + ra[q] * oc.row_stride) + k0;
-                const float* pb = reinterpret_cast<const float*>(orow.payload + rb[q] * orow.row_stride) + k0;
-                const float4 a0 = *reinterpret_cast<const float4*>(pa), a1 = *reinterpret_cast<const float4*>(pa + 4);
-                const float4 b0 = *reinterpret_cast<const float4*>(pb), b1 = *reinterpret_cast<const float4*>(pb + 4);
-                A[q][0] = a0.x; A[q][1] = a0.y; A[q][2] = a0.z; A[q][3] = a0.w;
-                A[q][4] = a1.x; A[q][5] = a1.y; A[q][6] = a1.z; A[q][7] = a1.w;
-                B[q][0] = b0.x; B[q][1] = b0.y; B[q][2] = b0.z; B[q][3] = b0.w;
-                B[q][4] = b1.x; B[q][5] = b1.y; B[q][6] = b1.z; B[q][7] = b1.w;
-            } else {
-                load8_f32(oc, ra[q], k0, A[q]);
-                load8_f32(orow, rb[q], k0, B[q]);
+            pa[q] = reinterpret_cast<const float*>(oc.payload + ra[q] * oc.row_stride) + 8 * h;
+            pb[q] = reinterpret_cast<const float*>(orow.payload + rb[q] * orow.row_stride) + 8 * h;
+        }
+        float4 A[QA][2], B[QA][2];
+#pragma unroll
+        for (int q = 0; q < QA; ++q) {
+            A[q][0] = *reinterpret_cast<const float4*>(pa[q]);
+            A[q][1] = *reinterpret_cast<const float4*>(pa[q] + 4);
+            B[q][0] = *reinterpret_cast<const float4*>(pb[q]);
+            B[q][1] = *reinterpret_cast<const float4*>(pb[q] + 4);
+        }
+        for (int kc = 0; kc < k; kc += 16) {
+            const int kn = kc + 16 < k ? kc + 16 : kc;       // the last chunk reloads itself
+            float4 An[QA][2], Bn[QA][2];
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                An[q][0] = *reinterpret_cast<const float4*>(pa[q] + kn);
+                An[q][1] = *reinterpret_cast<const float4*>(pa[q] + kn + 4);
+                Bn[q][0] = *reinterpret_cast<const float4*>(pb[q] + kn);
+                Bn[q][1] = *reinterpret_cast<const float4*>(pb[q] + kn + 4);
+            }
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+#pragma unroll
+                for (int a = 0; a < QA; ++a)
+#pragma unroll
+                    for (int b = 0; b < QA; ++b) {
+                        const float av = e < 4 ? A[b][0][e] : A[b][1][e - 4];
+                        const float bv = e < 4 ? B[a][0][e] : B[a][1][e - 4];
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[a][b], 0, 0, 0);
+                    }
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                A[q][0] = An[q][0]; A[q][1] = An[q][1];
+                B[q][0] = Bn[q][0]; B[q][1] = Bn[q][1];
             }
         }
+    } else {
+        // an FP32 operand against a lower format (rare): expanded per chunk
+        for (int kc = 0; kc < k; kc += 16) {
+            const int k0 = kc + 8 * h;
+            float A[QA][8], B[QA][8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
+            for (int q = 0; q < QA; ++q) { load8_f32(oc, ra[q], k0, A[q]); load8_f32(orow, rb[q], k0, B[q]); }
 #pragma unroll
-            for (int a = 0; a < QA; ++a)
+            for (int e = 0; e < 8; ++e)
 #pragma unroll
-                for (int b = 0; b < QA; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[b][e], B[a][e], acc[a][b], 0, 0, 0);
-    };
-    if (oc.prec == MPC_PREC_FP32 && orow.prec == MPC_PREC_FP32)
-        k_loop<KC, 16>(k, [&](int kc) { body(kc, std::true_type{}); });
-    else
-        k_loop<0, 16>(k, [&](int kc) { body(kc, std::false_type{}); });
+                for (int a = 0; a < QA; ++a)
+#pragma unroll
+                    for (int b = 0; b < QA; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x2f32(A[b][e], B[a][e], acc[a][b], 0, 0, 0);
+        }
+    }
 }
 
-// CLS >= 0: only pairs of that compute class (and, for the FP8 class, of the
-// element-format pair FA/FB) are computed; the others exit at once.
-// CLS < 0: every pair, its class chosen per workgroup (one launch for all).
+// The products of the P.np panels are summed in the accumulators (K = np*k)
+// and subtracted from C once.  CLS >= 0: only the panels whose pair on this
+// tile has that compute class (and, for the FP8 class, that element-format
+// pair FA/FB) are used, and a tile with none exits without touching C; a
+// tile mixing classes across panels is updated once per class launch.
+// CLS < 0: every panel, the class chosen per workgroup and panel.
 template <int QA, int CLS, int FA, int FB, int KC>
 __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_t m, int64_t ldc,
-                                                   int k, const uint8_t* __restrict__ payload,
-                                                   int64_t row_stride,
-                                                   const uint8_t* __restrict__ scales,
-                                                   int tile_rows,
-                                                   const uint8_t* __restrict__ prec_of_tile,
+                                                   int k, const mpc_syrk_panels P,
+                                                   int64_t row_stride, int tile_rows,
                                                    const mpc_small* __restrict__ sm, int64_t npair,
                                                    int64_t per_xcd, int64_t mt, int64_t tcol0) {
     constexpr int TILE = 64 * QA;
@@ -364,16 +428,7 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
     while (tj > 0 && start(tj) > gid) --tj;
     while (start(tj + 1) <= gid) ++tj;
     const int64_t ti = tj + (gid - start(tj));
-
-    operand_src orow, ocol;
-    orow.prec = prec_of_tile[(ti * TILE) / tile_rows];
-    ocol.prec = prec_of_tile[(tj * TILE) / tile_rows];
-    const int cls = pair_class(orow.prec, ocol.prec);
-    const int fa = mpc_dev_is_e4m3(ocol.prec) ? 0 : 1, fb = mpc_dev_is_e4m3(orow.prec) ? 0 : 1;
-    if constexpr (CLS >= 0) {
-        if (cls != CLS) return;                              // another class's launch
-        if (CLS == MPC_PREC_E4M3 && (fa != FA || fb != FB)) return;   // another format pair
-    }
+    const int64_t trow = (ti * TILE) / tile_rows, tcol = (tj * TILE) / tile_rows;
 
     const int wave = threadIdx.x >> 6, wr = wave >> 1, wc = wave & 1;
     const bool diag = ti == tj;
@@ -387,83 +442,136 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
     t.colbase = tj * TILE + 32 * QA * wc;
     t.diag = diag && wr == wc;     // only the diagonal quadrants need the r >= c mask
     if (t.rowbase >= m || t.colbase >= m) return;
-    orow.payload = ocol.payload = payload;
-    orow.scales = ocol.scales = scales;
-    orow.row_stride = ocol.row_stride = row_stride;
-    orow.kb = ocol.kb = k >> 5;
 
     v16f acc[QA][QA];
 #pragma unroll
     for (int a = 0; a < QA; ++a)
 #pragma unroll
-        for (int b = 0; b < QA; ++b) acc[a][b] = v16f{};
-    const int c = CLS >= 0 ? CLS : cls;
-    if (c == MPC_PREC_E4M3) {
-        if (CLS >= 0) mm_fp8<QA, FA, FB, KC>(ocol, orow, t, k, acc);
-        else if (fa == 0 && fb == 0) mm_fp8<QA, 0, 0, KC>(ocol, orow, t, k, acc);
-        else if (fa == 0) mm_fp8<QA, 0, 1, KC>(ocol, orow, t, k, acc);
-        else if (fb == 0) mm_fp8<QA, 1, 0, KC>(ocol, orow, t, k, acc);
-        else mm_fp8<QA, 1, 1, KC>(ocol, orow, t, k, acc);
-    } else if (c == MPC_PREC_BF16) {
-        mm_16<QA, true, KC>(ocol, orow, t, k, acc);
-    } else if (c == MPC_PREC_FP16) {
-        mm_16<QA, false, KC>(ocol, orow, t, k, acc);
-    } else {
-        mm_f32<QA, KC>(ocol, orow, t, k, acc);
+        for (int b2 = 0; b2 < QA; ++b2) acc[a][b2] = v16f{};
+    bool any = false;
+    for (int q = 0; q < P.np; ++q) {
+        const int slot = (P.p0 + q) % P.slots;
+        const int64_t roff = P.rows0 - (int64_t)q * P.nb;   // C row 0 within this panel
+        const uint8_t* prec = P.prec + slot * 1024 + roff / tile_rows;
+        operand_src orow, ocol;
+        orow.prec = prec[trow];
+        ocol.prec = prec[tcol];
+        orow.payload = ocol.payload = P.payload + slot * P.pstride + roff * row_stride;
+        orow.scales = ocol.scales = P.scales + slot * P.sstride + roff * (k >> 5);
+        orow.row_stride = ocol.row_stride = row_stride;
+        orow.kb = ocol.kb = k >> 5;
+        const int cls = pair_class(orow.prec, ocol.prec);
+        const int fa = mpc_dev_is_e4m3(ocol.prec) ? 0 : 1, fb = mpc_dev_is_e4m3(orow.prec) ? 0 : 1;
+        const bool f32pair = orow.prec == MPC_PREC_FP32 && ocol.prec == MPC_PREC_FP32;
+        if (CLS >= 0) {            // this launch computes one class (and format pair) only
+            if (cls != CLS || (CLS == MPC_PREC_E4M3 && (fa != FA || fb != FB)) ||
+                (CLS == MPC_PREC_FP32 && f32pair != (FA == 1)))
+                continue;
+        }
+        any = true;
+        const int c = CLS >= 0 ? CLS : cls;
+        if (c == MPC_PREC_E4M3) {
+            if (CLS >= 0) mm_fp8<QA, FA, FB, KC>(ocol, orow, t, k, acc);
+            else if (fa == 0 && fb == 0) mm_fp8<QA, 0, 0, KC>(ocol, orow, t, k, acc);
+            else if (fa == 0) mm_fp8<QA, 0, 1, KC>(ocol, orow, t, k, acc);
+            else if (fb == 0) mm_fp8<QA, 1, 0, KC>(ocol, orow, t, k, acc);
+            else mm_fp8<QA, 1, 1, KC>(ocol, orow, t, k, acc);
+        } else if (c == MPC_PREC_BF16) {
+            mm_16<QA, true, KC>(ocol, orow, t, k, acc);
+        } else if (c == MPC_PREC_FP16) {
+            mm_16<QA, false, KC>(ocol, orow, t, k, acc);
+        } else if (CLS == MPC_PREC_FP32) {
+            mm_f32<QA, KC, FA == 1>(ocol, orow, t, k, acc);
+        } else {                   // mixed-class launch: the K loop is unrolled (KC > 0)
+            mm_f32<QA, KC, false>(ocol, orow, t, k, acc);
+        }
     }
+    if (!any) return;              // no panel of this launch's class on this tile
     update_c<QA>(t, acc);
 }
 
 }  // namespace
 
-// C(lower) -= Q Q^T restricted to the tile columns [tcol0, tcol1) of C.
-// mixed = 0: tiles of MPC_SYRK_TILE, one launch per compute class present in
-// class_mask (and per FP8 element-format pair in fp8_mask; bit 0 E4M3
-// elements, bit 1 E5M2), the pairs of other launches exiting at once.
-// mixed = 1 or 2: tiles of 64 * mixed in ONE launch in which each workgroup
-// takes its own pair's class, so the classes run side by side (the block
-// column on the critical path of the look-ahead uses 64-row tiles: a 128-row
-// tile per workgroup leaves most of the chip idle there).
-int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
-                            const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
-                            int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
-                            unsigned class_mask, unsigned fp8_mask, int64_t tcol0, int64_t tcol1,
-                            int mixed) {
-    if (m <= 0) return 0;
+static int syrk_check(int64_t ldc, int k, int tile_rows) {
     if (ldc > (int64_t(1) << 22)) {       // 32-bit buffer offsets of a 128-column tile
         mpc_set_error("syrk: ldc %lld exceeds 2^22", (long long)ldc);
         return MPC_ERR_ARG;
     }
-    if (mixed < 0 || mixed > 2) {
-        mpc_set_error("syrk: mixed must be 0, 1 or 2 (got %d)", mixed);
-        return MPC_ERR_ARG;
-    }
-    const int TILE = mixed ? 64 * mixed : MPC_SYRK_TILE;
     if (k % 64 || tile_rows % MPC_SYRK_TILE) {
         mpc_set_error("syrk: k (%d) must be a multiple of 64 and tile_rows (%d) of 128", k,
                       tile_rows);
         return MPC_ERR_ARG;
     }
-    const int64_t mt = (m + TILE - 1) / TILE;
-    if (tcol1 < 0 || tcol1 > mt) tcol1 = mt;
-    if (tcol0 >= tcol1) return 0;
-    auto start = [&](int64_t c) { return c * mt - c * (c - 1) / 2; };
-    const int64_t npair = start(tcol1) - start(tcol0);
-    const int64_t per_xcd = (npair + 7) / 8;
-    dim3 grid((unsigned)(per_xcd * 8));
-#define MPC_SYRK_LAUNCH(QA, CLS, FA, FB, KC)                                                      \
-    hipLaunchKernelGGL((syrk_kernel<QA, CLS, FA, FB, KC>), grid, dim3(256), 0, st, C, m, ldc, k,  \
-                       payload, row_stride, scales, tile_rows, prec_of_tile, sm, npair, per_xcd, \
-                       mt, tcol0)
-    if (mixed == 1) {             // latency-bound: the K loop unrolled for the usual k
+    return 0;
+}
+
+// launch geometry over tile columns [tcol0, tcol1) of the lower triangle
+struct syrk_grid {
+    int64_t mt, npair, per_xcd, tcol0;
+    dim3 grid;
+};
+static bool syrk_geometry(int64_t m, int tile, int64_t tcol0, int64_t tcol1, syrk_grid& g) {
+    g.mt = (m + tile - 1) / tile;
+    if (tcol1 < 0 || tcol1 > g.mt) tcol1 = g.mt;
+    if (tcol0 >= tcol1) return false;
+    auto start = [&](int64_t c) { return c * g.mt - c * (c - 1) / 2; };
+    g.npair = start(tcol1) - start(tcol0);
+    g.per_xcd = (g.npair + 7) / 8;
+    g.tcol0 = tcol0;
+    g.grid = dim3((unsigned)(g.per_xcd * 8));
+    return true;
+}
+
+#define MPC_SYRK_LAUNCH(QA, CLS, FA, FB, KC)                                                   \
+    hipLaunchKernelGGL((syrk_kernel<QA, CLS, FA, FB, KC>), g.grid, dim3(256), 0, st, C, m, ldc, \
+                       k, P, row_stride, tile_rows, sm, g.npair, g.per_xcd, g.mt, g.tcol0)
+
+// C(lower) -= Q Q^T restricted to the tile columns [tcol0, tcol1) of C, one
+// panel.  mixed = 0: tiles of MPC_SYRK_TILE, one launch per compute class
+// present in class_mask (and per FP8 element-format pair in fp8_mask; bit 0
+// E4M3 elements, bit 1 E5M2).  mixed = 1: 64-row tiles in ONE launch in which
+// each workgroup takes its own pair's class (the block column on the
+// critical path of the look-ahead: a launch per class would serialise, and a
+// 128-row tile per workgroup leaves most of the chip idle there).
+int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
+                            const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
+                            int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
+                            unsigned class_mask, unsigned fp8_mask, int64_t tcol0, int64_t tcol1,
+                            int mixed) {
+    if (mixed != 0 && mixed != 1) {
+        mpc_set_error("syrk: mixed must be 0 or 1 (got %d)", mixed);
+        return MPC_ERR_ARG;
+    }
+    mpc_syrk_panels P{};
+    P.payload = payload;
+    P.scales = scales;
+    P.prec = prec_of_tile;
+    P.slots = 1;
+    P.np = 1;
+    P.nb = tile_rows;
+    return mpc_launch_syrk_panels(st, C, m, ldc, k, P, row_stride, tile_rows, sm, tcol0, tcol1,
+                                  mixed ? 1 : 2, class_mask, fp8_mask);
+}
+
+int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
+                           const mpc_syrk_panels& P, int64_t row_stride, int tile_rows,
+                           const mpc_small* sm, int64_t tcol0, int64_t tcol1, int qa,
+                           unsigned class_mask, unsigned fp8_mask) {
+    if (m <= 0) return 0;
+    if ((qa != 1 && qa != 2) || P.np < 1 || P.np > MPC_SYRK_MAXP || P.slots < 1 ||
+        P.rows0 - (int64_t)(P.np - 1) * P.nb < 0 || P.rows0 % tile_rows || P.nb % tile_rows) {
+        mpc_set_error("syrk: bad panel set (qa %d, np %d of at most %d, rows0 %lld, nb %d)", qa,
+                      P.np, MPC_SYRK_MAXP, (long long)P.rows0, P.nb);
+        return MPC_ERR_ARG;
+    }
+    int rc = syrk_check(ldc, k, tile_rows);
+    if (rc) return rc;
+    syrk_grid g;
+    if (!syrk_geometry(m, 64 * qa, tcol0, tcol1, g)) return 0;
+    if (qa == 1) {                // latency-bound: the K loop unrolled for the usual k
         if (k == 256) MPC_SYRK_LAUNCH(1, -1, 0, 0, 256);
         else if (k == 128) MPC_SYRK_LAUNCH(1, -1, 0, 0, 128);
         else MPC_SYRK_LAUNCH(1, -1, 0, 0, 0);
-        MPC_LAUNCH_CHECK();
-        return 0;
-    }
-    if (mixed == 2) {
-        MPC_SYRK_LAUNCH(2, -1, 0, 0, 0);
         MPC_LAUNCH_CHECK();
         return 0;
     }
@@ -475,11 +583,15 @@ int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, in
     }
     if (class_mask & (1u << MPC_PREC_BF16)) MPC_SYRK_LAUNCH(2, MPC_PREC_BF16, 0, 0, 0);
     if (class_mask & (1u << MPC_PREC_FP16)) MPC_SYRK_LAUNCH(2, MPC_PREC_FP16, 0, 0, 0);
-    if (class_mask & (1u << MPC_PREC_FP32)) MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 0, 0, 0);
-#undef MPC_SYRK_LAUNCH
+    if (class_mask & (1u << MPC_PREC_FP32)) {
+        MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 1, 0, 0);          // FP32 x FP32
+        if (class_mask & ~(1u << MPC_PREC_FP32))              // FP32 x a lower format
+            MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 0, 0, 0);
+    }
     MPC_LAUNCH_CHECK();
     return 0;
 }
+#undef MPC_SYRK_LAUNCH
 
 int mpc_launch_syrk(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                     const uint8_t* payload, int64_t row_stride, const uint8_t* scales,

@@ -69,6 +69,9 @@ struct mpc_handle {
     hipEvent_t   ev_col = nullptr;     // factor start (on stream)
     hipEvent_t   ev_panel = nullptr;   // next panel ready (on pstream)
     hipEvent_t   ev_bulk = nullptr;    // bulk update of a step done (on stream)
+    // aggregated updates of the block columns beyond the next group's window
+    hipStream_t  fstream = nullptr;
+    hipEvent_t   ev_far = nullptr;     // an aggregated far update done (on fstream)
     // big-block inverses of L for the PCG sweeps, built on wstream during the
     // factor as each big diagonal block becomes final
     hipStream_t  wstream = nullptr;
@@ -127,8 +130,13 @@ struct mpc_small {
     // trailing update of panel j still reads panel j's buffers
     uint32_t tile_amax[2][1024];   // bits of positive floats (atomicMax)
     uint32_t tile_maxrs[2][1024];  // max row sum of squares
-    uint8_t  tile_prec[2][1024];   // decided precision per panel tile
+    // decided precision per panel tile, one slot per panel buffer (a panel
+    // stays live until the aggregated update of its group has read it)
+    uint8_t  tile_prec[2 * 4][1024];
 };
+constexpr int MPC_AGG_MAX = 4;                    // panels per aggregated update
+constexpr int MPC_PANEL_SLOTS = 2 * MPC_AGG_MAX;  // panel buffers in the ring
+static_assert(sizeof(((mpc_small*)nullptr)->tile_prec) / 1024 == MPC_PANEL_SLOTS, "slots");
 
 // ---------------------------------------------------------------- kernels (launchers)
 // k_potrf.hip
@@ -152,11 +160,33 @@ int mpc_launch_syrk(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                     const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                     int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm);
 constexpr int MPC_SYRK_TILE = 128;   // output tile edge of the SYRK kernels
+// Several quantised panels whose products are summed into one update
+// (K = np * k).  The panels sit in a ring of `slots` buffers (payload,
+// scales, 1024 tile precisions each); panel q of the update (q < np) is
+// panel p0 + q, in slot (p0 + q) % slots, and C's row 0 is its row
+// rows0 - q * nb (each panel starts nb rows below the previous one).
+// Scalars only: a kernel argument of pointer arrays costs scalar registers.
+constexpr int MPC_SYRK_MAXP = 8;
+struct mpc_syrk_panels {
+    const uint8_t* payload;
+    const uint8_t* scales;
+    const uint8_t* prec;
+    int64_t pstride, sstride;    // bytes between slots (prec: 1024 per slot)
+    int64_t rows0;
+    int slots, p0, np, nb;
+};
+// C(lower) -= sum_p Q_p Q_p^T over tile columns [tcol0, tcol1), tiles of
+// 64 * qa rows: qa = 1 one launch with the class chosen per workgroup and
+// panel; qa = 2 one launch per class in class_mask / fp8 pair in fp8_mask
+int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
+                           const mpc_syrk_panels& P, int64_t row_stride, int tile_rows,
+                           const mpc_small* sm, int64_t tcol0, int64_t tcol1, int qa,
+                           unsigned class_mask = 0xffu, unsigned fp8_mask = 3u);
 int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                             const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                             int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
                             unsigned class_mask, unsigned fp8_mask, int64_t tcol0 = 0,
-                            int64_t tcol1 = -1, int mixed = 0);
+                            int64_t tcol1 = -1, int mixed = 0);   // mixed: 0 or 1
 // compute classes / FP8 element formats that a factor configuration can produce
 void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* class_mask,
                     unsigned* fp8_mask);

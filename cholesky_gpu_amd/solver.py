@@ -62,7 +62,7 @@ def _pre(h):
 
 # ---------------------------------------------------------------- options
 def factor_opts(nb=256, update_prec=L.PREC_SWITCH, ladder=(L.PREC_E4M3, L.PREC_FP32),
-                eps_prime=0.25, fallback=1, profile=0, update_mixed=0):
+                eps_prime=0.25, fallback=1, profile=0, agg_panels=None):
     o = L.FactorOpts()
     L.load().mpc_factor_opts_default(C.byref(o))
     o.nb = nb
@@ -73,7 +73,8 @@ def factor_opts(nb=256, update_prec=L.PREC_SWITCH, ladder=(L.PREC_E4M3, L.PREC_F
     o.eps_prime = eps_prime
     o.fallback = fallback
     o.profile = profile
-    o.update_mixed = update_mixed
+    if agg_panels is not None:
+        o.agg_panels = agg_panels
     return o
 
 

@@ -126,9 +126,9 @@ typedef struct {
     int    fallback;        /* 1: on a non-positive pivot restore A, divide
                                eps_prime by 16 and retry (twice), then all-FP32 */
     int    profile;         /* 1: time the phases with events (adds syncs) */
-    int    update_mixed;    /* trailing update beyond the look-ahead column:
-                               0 = one launch per precision class (default),
-                               1 = one launch, classes side by side */
+    int    agg_panels;      /* panels whose trailing updates beyond their group
+                               of block columns are applied together
+                               (K = agg_panels * nb): 1..4, default 2 */
 } mpc_factor_opts;
 /* Rule (per panel tile L_ij, i > j): the lowest-precision ladder entry p with
        u_p * max_{r in tile} sum_k L_rk^2  <=  eps_prime * min_k L_kk^2,
