@@ -23,7 +23,6 @@ constexpr int SUBLD = 33;                 // padded sub-tile column stride
 constexpr int SUBSZ = 32 * SUBLD;         // floats per sub-tile
 constexpr int MAXSB = 8;                  // 256 / 32
 constexpr int NSUB = MAXSB * (MAXSB + 1) / 2;
-constexpr int LRLD = 36;                  // row stride of Lr (16-byte aligned rows)
 // Four waves (one per SIMD) at about 200 registers each: the factor then
 // fits on a CU beside one workgroup of the concurrent trailing update.
 // Eight waves are faster alone but need a whole CU, which the update's
@@ -35,8 +34,7 @@ constexpr int NT = 64 * NW;
 constexpr bool kPotrfRefine = false;
 
 typedef float v16f __attribute__((ext_vector_type(16)));
-// LDS (address space 3) pointers: an opaque copy stays a 32-bit LDS address
-typedef const __attribute__((address_space(3))) float* lds_cfloat_p;
+// LDS (address space 3) pointer: an opaque copy stays a 32-bit LDS address
 typedef float f4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(3))) f4* lds_cf4_p;
 
@@ -47,6 +45,16 @@ __device__ __forceinline__ int sig(int i, int h) { return 8 * (i >> 2) + 4 * h +
 
 __device__ __forceinline__ float rdlane(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// lane l of half h (lanes 32 h .. 32 h + 31) to lanes l and l + 32: the half
+// swap exchanges lanes 32..63 of its first operand with lanes 0..31 of its
+// second; with both operands v, the first result holds the low half twice
+// and the second the high half twice
+__device__ __forceinline__ float bcast_half(float v, int h) {
+    const auto p = __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false,
+                                                    false);
+    return __uint_as_float(h ? p[1] : p[0]);
 }
 
 // D[m][n] (32x32) += A[m][k] * B[k][n] over k = 0..31 with both operands in LDS:
@@ -140,8 +148,8 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                                                     uint32_t* __restrict__ st_maxrs) {
     __shared__ float T[NSUB * SUBSZ];
     __shared__ float Dinv[SUBSZ];
-    __shared__ float colb[2][32];                   // column k of L, broadcast
-    __shared__ __align__(16) float Lr[33 * LRLD];   // row-major sub-tile; row 32: 1 / L_kk
+    __shared__ __align__(16) float colb[2][32];     // column k of L by row parity
+    __shared__ __align__(16) float Lc[32 * 32];     // L's columns, rows split by parity
     __shared__ int fail_col;
     __shared__ float red[256];
 
@@ -202,33 +210,37 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
             store_subcol(T, A, lda, bs, kb - 1, 64, NT - 64);
         }
         if (wave == 0) {
-            // Pipelined right-looking factor, row per lane (lanes 32..63 mirror
-            // 0..31).  Column k+1 receives column k's update through one
-            // readlane (the pivot chain); the other columns receive it one step
+            // Pipelined right-looking factor, row per lane, the row's columns
+            // split over the two lane halves by parity: lane r + 32 h holds
+            // e[u] = A[r][2u + h].  Column k+1 receives column k's update at
+            // once (the pivot chain); the other columns receive it one step
             // later from an LDS broadcast read right after it is written, so
-            // the read latency overlaps the next pivot.  The opaque copy
-            // of the lane id keeps the lane-dependent predicates inside the kb
-            // loop (hoisted out of it they exhaust the scalar registers).
+            // the read latency overlaps the next pivot.  Column k's entries
+            // reach the other half through a half swap (one VALU op).
+            // Updates are not predicated on r >= c: entries above the
+            // diagonal only accumulate values no valid entry ever reads (they
+            // are not broadcast, not pivots, and zeroed before the row is
+            // written back).
             int r = lane & 31;
             asm volatile("" : "+v"(r));
-            // Updates are not predicated on r >= c: entries above the diagonal
-            // only accumulate values no valid entry ever reads (they are not
-            // broadcast, not pivots, and zeroed when the row is written back).
-            float d[32];
+            const bool hi = lane >= 32;
+            float e[16];
 #pragma unroll
-            for (int c = 0; c < 32; ++c) {
+            for (int u = 0; u < 16; ++u) {
+                const int c = 2 * u + (hi ? 1 : 0);
                 const float v = D[r + c * SUBLD];
-                d[c] = (c <= r) ? v : 0.f;
+                e[u] = (c <= r) ? v : 0.f;
             }
-            int bad = -1;
-            float lc[32];          // column k-1 of L, prefetched from the broadcast
+            float lc[16];          // column k-1 of L at this lane's columns
+            float lprev = 0.f;     // L[r][k-1]
+            float rv = 1.f;        // lane r: 1 / L[r][r]
+            float pv = 1.f;        // lane r: the pivot of column r as met
 #pragma unroll
             for (int k = 0; k < 32; ++k) {
-                float piv = rdlane(d[k], k);
-                if (!(piv > 0.f)) {                  // uniform across the wave
-                    if (bad < 0) bad = k;
-                    piv = 1.f;
-                }
+                const int hk = k & 1, uk = k >> 1;
+                const float pr = rdlane(e[uk], k + 32 * hk);
+                pv = (r == k) ? pr : pv;
+                const float piv = (pr > 0.f) ? pr : 1.f;
                 // 1/sqrt(piv) from the hardware estimate and one Newton step
                 // (about 1 ulp), sqrt(piv) = piv / sqrt(piv): a short
                 // dependent chain per pivot, where a correctly rounded sqrt
@@ -237,72 +249,116 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                 const float hr = 0.5f * piv * rl;
                 rl = fmaf(fmaf(-hr, rl, 0.5f), rl, rl);
                 const float l = piv * rl;
-                d[k] = (r == k) ? l : d[k] * rl;
-                if (r == k) Lr[32 * LRLD + k] = rl;
+                rv = (r == k) ? rl : rv;
+                const float ek = (r == k) ? l : e[uk] * rl;    // valid in half hk
+                e[uk] = (hi == (hk == 1)) ? ek : e[uk];
+                const float lk = bcast_half(ek, hk);           // L[r][k], both halves
+                colb[k & 1][(r & 1) * 16 + (r >> 1)] = lk;
                 if (k < 31) {
-                    if (lane < 32) colb[k & 1][r] = d[k];
-                    const float v = rdlane(d[k], k + 1);   // l_{k+1,k}: the pivot chain
-                    d[k + 1] = fmaf(-d[k], v, d[k + 1]);
+                    const int h1 = (k + 1) & 1, u1 = (k + 1) >> 1;
+                    const float v = rdlane(ek, (k + 1) + 32 * hk);   // L[k+1][k]
+                    const float t = fmaf(-lk, v, e[u1]);
+                    e[u1] = (hi == (h1 == 1)) ? t : e[u1];
                 }
                 if (k >= 1) {                         // deferred: column k-1 -> columns > k
-                    const float lr = d[k - 1];
 #pragma unroll
-                    for (int c = k + 1; c < 32; ++c) d[c] = fmaf(-lr, lc[c], d[c]);
+                    for (int u = (k >> 1) + 1; u < 16; ++u) e[u] = fmaf(-lprev, lc[u], e[u]);
+                    if ((k & 1) == 0) {               // column k+1 = 2 (k/2) + 1: half 1 only
+                        const int u = k >> 1;
+                        const float t = fmaf(-lprev, lc[u], e[u]);
+                        e[u] = hi ? t : e[u];
+                    }
                 }
                 if (k < 30) {                         // prefetch column k for step k+1
+                    const f4* cb = reinterpret_cast<const f4*>(&colb[k & 1][hi ? 16 : 0]);
 #pragma unroll
-                    for (int c = k + 2; c < 32; ++c) lc[c] = colb[k & 1][c];
+                    for (int g = ((k >> 1) + 1) >> 2; g < 4; ++g) {
+                        const f4 t = cb[g];
+                        lc[4 * g] = t.x; lc[4 * g + 1] = t.y; lc[4 * g + 2] = t.z; lc[4 * g + 3] = t.w;
+                    }
+                }
+                lprev = lk;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) e[u] = (2 * u + (hi ? 1 : 0) <= r) ? e[u] : 0.f;
+            // the whole sub-tile, zeros above the diagonal (the trailing
+            // updates of earlier kb wrote the full 32x32 block); Lc keeps the
+            // columns of L split by row parity for the inverse's broadcast
+            // reads: Lc[q][h][v] = L[2v + h][q]
+            {
+                const int lo = (r & 1) * 16 + (r >> 1);
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int c = 2 * u + (hi ? 1 : 0);
+                    D[r + c * SUBLD] = e[u];
+                    Lc[c * 32 + lo] = e[u];
                 }
             }
-#pragma unroll
-            for (int c = 0; c < 32; ++c) d[c] = (c <= r) ? d[c] : 0.f;
-            // the whole sub-tile, zeros above the diagonal: the trailing
-            // updates of earlier kb wrote the full 32x32 block, and the
-            // refinement below reads D as a lower-triangular operand; Lr keeps
-            // a row-major copy for the inverse's broadcast reads
-            if (lane < 32) {
-#pragma unroll
-                for (int c = 0; c < 32; ++c) D[r + c * SUBLD] = d[c];
-#pragma unroll
-                for (int c = 0; c < 32; c += 4)
-                    *reinterpret_cast<float4*>(&Lr[r * LRLD + c]) =
-                        make_float4(d[c], d[c + 1], d[c + 2], d[c + 3]);
+            {
+                const uint64_t badm = __builtin_amdgcn_ballot_w64(!(pv > 0.f)) & 0xffffffffull;
+                const int bad = badm ? __builtin_ctzll(badm) : -1;
+                if (bad >= 0 && lane == 0 && kb * 32 + bad < bs && fail_col < 0)
+                    fail_col = kb * 32 + bad;
             }
-            if (bad >= 0 && lane == 0 && kb * 32 + bad < bs && fail_col < 0)
-                fail_col = kb * 32 + bad;
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            // inverse, column per lane: x[p] = (e_c[p] - sum_{q<p} L[p][q] x[q]) / L[p][p],
-            // row p of L read by every lane at the same LDS address (broadcast)
-            // (Lr lies beyond the 64 KB reach of the LDS instruction offset:
-            // an opaque base keeps every read at base + immediate offset)
+            // inverse, right-looking, column c = r per lane, rows split over
+            // the halves by parity (acc[v] = X[2v + h][c]):
+            //   x_q = acc_q / L[q][q];  acc_p -= L[p][q] x_q  (p > q)
+            // x_q reaches the other half through a half swap; column q of L
+            // is read from Lc by both halves (one address per half), one step
+            // ahead.
             const int c = r;
-            lds_cfloat_p lrb = (lds_cfloat_p)(Lr);
-            asm volatile("" : "+v"(lrb));
-            float x[32];
+            float acc[16];
 #pragma unroll
-            for (int p = 0; p < 32; ++p) {
-                // row p's reads wait for x[p-2]: without this order the
-                // scheduler hoists all 528 reads of L to the top, which
-                // exhausts the registers and spills to scratch memory
-                if (p >= 2) asm volatile("" : "+v"(lrb) : "v"(x[p - 2]));
-                float a0 = (p == c) ? 1.f : 0.f, a1 = 0.f;
+            for (int v = 0; v < 16; ++v) acc[v] = (2 * v + (hi ? 1 : 0) == c) ? 1.f : 0.f;
+            lds_cf4_p lcb = (lds_cf4_p)(Lc + (hi ? 16 : 0));
+            asm volatile("" : "+v"(lcb));
+            f4 lq[4];
 #pragma unroll
-                for (int q4 = 0; q4 < p; q4 += 4) {
-                    const f4 lp = *(lds_cf4_p)(&lrb[p * LRLD + q4]);
-                    a0 = fmaf(-lp.x, x[q4], a0);
-                    if (q4 + 1 < p) a1 = fmaf(-lp.y, x[q4 + 1], a1);
-                    if (q4 + 2 < p) a0 = fmaf(-lp.z, x[q4 + 2], a0);
-                    if (q4 + 3 < p) a1 = fmaf(-lp.w, x[q4 + 3], a1);
+            for (int g = 0; g < 4; ++g) lq[g] = lcb[g];
+            float xprev = 0.f;
+#pragma unroll
+            for (int q = 0; q < 32; ++q) {
+                const int hq = q & 1, vq = q >> 1;
+                // the next column's reads wait for x_{q-1}: without this
+                // order the scheduler hoists the reads of all 32 columns to
+                // the top, which exhausts the registers
+                f4 ln[4];
+                if (q < 31) {
+                    asm volatile("" : "+v"(lcb) : "v"(xprev));
+#pragma unroll
+                    for (int g = ((q + 1) >> 1) >> 2; g < 4; ++g) ln[g] = lcb[(q + 1) * 8 + g];
                 }
-                x[p] = (a0 + a1) * lrb[32 * LRLD + p];
+                const float rq = rdlane(rv, q);
+                const float xq = acc[vq] * rq;                 // valid in half hq
+                acc[vq] = (hi == (hq == 1)) ? xq : acc[vq];
+                const float xb = bcast_half(xq, hq);
+                float Lq[16];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    Lq[4 * g] = lq[g].x; Lq[4 * g + 1] = lq[g].y;
+                    Lq[4 * g + 2] = lq[g].z; Lq[4 * g + 3] = lq[g].w;
+                }
+#pragma unroll
+                for (int v = vq + 1; v < 16; ++v) acc[v] = fmaf(-Lq[v], xb, acc[v]);
+                if (hq == 0) {                                 // row q + 1 = 2 vq + 1: half 1
+                    const float t = fmaf(-Lq[vq], xb, acc[vq]);
+                    acc[vq] = hi ? t : acc[vq];
+                }
+                if (q < 31) {
+#pragma unroll
+                    for (int g = ((q + 1) >> 1) >> 2; g < 4; ++g) lq[g] = ln[g];
+                }
+                xprev = xb;
             }
-            if (lane < 32) {
+            {
                 float* gd = dinv + (int64_t)((col0 >> 5) + kb) * 1024;
 #pragma unroll
-                for (int p = 0; p < 32; ++p) {
-                    Dinv[p + c * SUBLD] = x[p];
-                    if (kb * 32 < bs) gd[p + c * 32] = x[p];
+                for (int v = 0; v < 16; ++v) {
+                    const int p = 2 * v + (hi ? 1 : 0);
+                    Dinv[p + c * SUBLD] = acc[v];
+                    if (kb * 32 < bs) gd[p + c * 32] = acc[v];
                 }
             }
         }
