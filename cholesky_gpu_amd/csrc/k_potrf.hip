@@ -60,16 +60,22 @@ __device__ __forceinline__ float bcast_half(float v, int h) {
 // D[m][n] (32x32) += A[m][k] * B[k][n] over k = 0..31 with both operands in LDS:
 // A(m,k) = a[m + k*lda_], B(k,n) = b[n + k*ldb_] (i.e. B stored as rows n).
 __device__ __forceinline__ v16f mfma_32x32x32_lds(v16f acc, const float* a, int lda_,
-                                                 const float* b, int ldb_, float asign) {
+                                                 const float* b, int ldb_) {
     const int lane = threadIdx.x & 63;
     const int m = lane & 31, h = lane >> 5;
+    // every operand read issued before the first product: interleaved, each
+    // pair of products waits a full LDS round trip
+    float av[16], bv[16];
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
         const int k = 2 * s + h;
-        float av = asign * a[m + k * lda_];
-        float bv = b[m + k * ldb_];
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+        av[s] = a[m + k * lda_];
+        bv[s] = b[m + k * ldb_];
     }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s = 0; s < 16; ++s)
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[s], bv[s], acc, 0, 0, 0);
     return acc;
 }
 
@@ -80,13 +86,15 @@ __device__ __forceinline__ void update_subtile(float* T, int ib, int jb, int kb)
     const float* Ai = T + sidx(ib, kb) * SUBSZ;
     const float* Bj = T + sidx(jb, kb) * SUBSZ;
     const int n = lane & 31, h = lane >> 5;
+    // -C += Ai Bj^T, negated on the way in and out (a negated operand would
+    // put a dependent VALU op between consecutive products)
     v16f acc;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) acc[i] = Cs[sig(i, h) + n * SUBLD];
-    // A(m=r,k=q) = -Ai(r,q); B(k=q,n=c) = Bj(c,q)
-    acc = mfma_32x32x32_lds(acc, Ai, SUBLD, Bj, SUBLD, -1.f);
+    for (int i = 0; i < 16; ++i) acc[i] = -Cs[sig(i, h) + n * SUBLD];
+    // A(m=r,k=q) = Ai(r,q); B(k=q,n=c) = Bj(c,q)
+    acc = mfma_32x32x32_lds(acc, Ai, SUBLD, Bj, SUBLD);
 #pragma unroll
-    for (int i = 0; i < 16; ++i) Cs[sig(i, h) + n * SUBLD] = acc[i];
+    for (int i = 0; i < 16; ++i) Cs[sig(i, h) + n * SUBLD] = -acc[i];
 }
 
 // Columns [c_lo, c_hi) of the lower block into the sub-tiles (identity
@@ -128,16 +136,95 @@ __device__ __forceinline__ void load_cols_vec(float* T, const float* __restrict_
     }
 }
 
+// The lower sub-tiles right of sub-column 0 ((ib, cb), 1 <= cb <= ib < nsb,
+// numbered by sub-column, then row) into T (identity beyond bs, zeros above
+// the diagonal), 16-byte path: wave w of nwav reads tiles w, w + nwav, ...,
+// TB of them per batch with all loads of a batch in flight; in a tile, load q
+// of lane l covers column (l + 64 q) / 8 and rows 4 ((l + 64 q) % 8) .. +3
+// (eight lanes per 128-byte column segment).  Only the lower part is read.
+template <int TB>
+__device__ __forceinline__ void load_tiles_vec(float* T, const float* __restrict__ A, int64_t lda,
+                                               int bs, int nsb, int w, int nwav) {
+    const int lane = threadIdx.x & 63;
+    const int ntile = nsb * (nsb - 1) / 2;
+    for (int t0 = w; t0 < ntile; t0 += nwav * TB) {
+        float4 v[TB][4];
+        int tib[TB], tcb[TB];
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+            // clamped tile index and addresses: unconditional loads (a load
+            // under a condition becomes a branch with its own wait)
+            int rem = t0 + nwav * u < ntile ? t0 + nwav * u : ntile - 1, cb = 1;
+            while (rem >= nsb - cb) {
+                rem -= nsb - cb;
+                ++cb;
+            }
+            tib[u] = cb + rem;
+            tcb[u] = cb;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = lane + 64 * q;
+                const int c = cb * 32 + (j >> 3), r4 = tib[u] * 32 + (j & 7) * 4;
+                const int cc = c < bs ? c : bs - 1, rc = r4 < bs ? r4 : bs - 4;
+                v[u][q] = *reinterpret_cast<const float4*>(A + rc + (int64_t)cc * lda);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+            if (t0 + nwav * u >= ntile) continue;
+            float* Ts = T + sidx(tib[u], tcb[u]) * SUBSZ;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int j = lane + 64 * q;
+                const int c = tcb[u] * 32 + (j >> 3), r4 = tib[u] * 32 + (j & 7) * 4;
+                const float e[4] = {v[u][q].x, v[u][q].y, v[u][q].z, v[u][q].w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int r = r4 + i;
+                    float x = (r < c) ? 0.f : e[i];
+                    if (r >= bs || c >= bs) x = (r == c) ? 1.f : 0.f;
+                    Ts[(r & 31) + (c & 31) * SUBLD] = x;
+                }
+            }
+        }
+    }
+}
+
 // Sub-column cb (its lower part, rows cb*32 .. bs-1) back to A by threads
 // t0 .. t0+nthr-1: a thread per row, consecutive threads consecutive rows.
 __device__ __forceinline__ void store_subcol(const float* T, float* __restrict__ A, int64_t lda,
                                              int bs, int cb, int t0, int nthr) {
     const int c0 = cb * 32, c1 = bs < c0 + 32 ? bs : c0 + 32;
+    // byte offsets from A in one 32-bit VGPR per store (a 64-bit address per
+    // store would hold twice the registers); the launch checks that the
+    // block's bs + bs lda elements stay below 2^31 bytes
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(A, 0, 0x7fffffff,
+                                                                          0x00020000);
+    const uint32_t ld32 = (uint32_t)lda;
     for (int r = c0 + (int)threadIdx.x - t0; r < bs; r += nthr) {
-        const int cend = r + 1 < c1 ? r + 1 : c1;        // lower triangle only
-        const float* Ts = T + sidx(r >> 5, cb) * SUBSZ + (r & 31);
-        float* Ar = A + r;
-        for (int c = c0; c < cend; ++c) Ar[(int64_t)c * lda] = Ts[(c & 31) * SUBLD];
+        // Every (r, c) is stored unconditionally: where c > r (diagonal
+        // sub-tile) or c >= bs, the store goes to the lower-triangle entry
+        // (max, min) of (r, min(c, c1 - 1)) with that entry's value, which
+        // its own row also stores (the same value twice).  A per-column
+        // predicate instead keeps 32 lane masks alive.  The row's 32 reads
+        // are in flight at once, then the stores.
+        const float* Tc = T + sidx(r >> 5, cb) * SUBSZ;
+#pragma unroll
+        for (int c8 = 0; c8 < 32; c8 += 16) {
+            float v[16];
+            uint32_t off[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const int cc = c0 + c8 + c < c1 ? c0 + c8 + c : c1 - 1;
+                const int rs = r > cc ? r : cc, cs = r > cc ? cc : r;
+                v[c] = Tc[(rs & 31) + (cs & 31) * SUBLD];
+                off[c] = 4u * ((uint32_t)rs + (uint32_t)cs * ld32);
+            }
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[c]), rsrc, off[c], 0, 0);
+            __builtin_amdgcn_sched_barrier(0);      // halves kept apart (registers)
+        }
     }
 }
 
@@ -196,7 +283,7 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
         // waves 1..3 finish the trailing update of kb-1 on the sub-columns
         // beyond kb (3b below), which the factor of sub-block kb does not read
         if (wave != 0 && kb == 0 && vec)                 // the rest of the block
-            load_cols_vec<16>(T, A, lda, bs, nsb, 32, nsb * 32, wave - 1, NW - 1);
+            load_tiles_vec<4>(T, A, lda, bs, nsb, wave - 1, NW - 1);
         if (wave != 0 && kb >= 1) {
             const int m = nsb - 1 - kb;                  // sub-columns kb+1 .. nsb-1
             const int npair = m * (m + 1) / 2;
@@ -373,12 +460,16 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
             float* S = T + sidx(ib, kb) * SUBSZ;
             const int n = lane & 31, h = lane >> 5;
             v16f s0, x = {};
+            float dv[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) s0[i] = S[n + sig(i, h) * SUBLD];
+            for (int i = 0; i < 16; ++i) {
+                s0[i] = S[n + sig(i, h) * SUBLD];
+                dv[i] = Dinv[n + sig(i, h) * SUBLD];
+            }
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int t = 0; t < 16; ++t)
-                x = __builtin_amdgcn_mfma_f32_32x32x2f32(Dinv[n + sig(t, h) * SUBLD], s0[t], x,
-                                                         0, 0, 0);
+                x = __builtin_amdgcn_mfma_f32_32x32x2f32(dv[t], s0[t], x, 0, 0, 0);
             if constexpr (kPotrfRefine) {
                 v16f r = s0;
 #pragma unroll
@@ -431,6 +522,10 @@ int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0, fl
                      mpc_small* sm, int ntiles_next, uint32_t* stats_amax, uint32_t* stats_maxrs) {
     if (bs <= 0 || bs > 256) {
         mpc_set_error("potrf: block size %d outside 1..256", bs);
+        return MPC_ERR_ARG;
+    }
+    if (lda > (1 << 21)) {
+        mpc_set_error("potrf: leading dimension %lld above 2^21", (long long)lda);
         return MPC_ERR_ARG;
     }
     if (ntiles_next > 1024) {
