@@ -318,7 +318,7 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                 const float v = D[r + c * SUBLD];
                 e[u] = (c <= r) ? v : 0.f;
             }
-            float lc[16];          // column k-1 of L at this lane's columns
+            f4 lcg[4];             // column k-1 of L at this lane's columns, by 4
             float lprev = 0.f;     // L[r][k-1]
             float rv = 1.f;        // lane r: 1 / L[r][r]
             float pv = 1.f;        // lane r: the pivot of column r as met
@@ -348,6 +348,16 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                     e[u1] = (hi == (h1 == 1)) ? t : e[u1];
                 }
                 if (k >= 1) {                         // deferred: column k-1 -> columns > k
+                    // whole groups live until here: a dead component's
+                    // register reused early would wait for the read in flight
+#pragma unroll
+                    for (int g = (k >> 1) >> 2; g < 4; ++g) asm volatile("" : "+v"(lcg[g]));
+                    float lc[16];
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        lc[4 * g] = lcg[g].x; lc[4 * g + 1] = lcg[g].y;
+                        lc[4 * g + 2] = lcg[g].z; lc[4 * g + 3] = lcg[g].w;
+                    }
 #pragma unroll
                     for (int u = (k >> 1) + 1; u < 16; ++u) e[u] = fmaf(-lprev, lc[u], e[u]);
                     if ((k & 1) == 0) {               // column k+1 = 2 (k/2) + 1: half 1 only
@@ -359,10 +369,10 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                 if (k < 30) {                         // prefetch column k for step k+1
                     const f4* cb = reinterpret_cast<const f4*>(&colb[k & 1][hi ? 16 : 0]);
 #pragma unroll
-                    for (int g = ((k >> 1) + 1) >> 2; g < 4; ++g) {
-                        const f4 t = cb[g];
-                        lc[4 * g] = t.x; lc[4 * g + 1] = t.y; lc[4 * g + 2] = t.z; lc[4 * g + 3] = t.w;
-                    }
+                    for (int g = ((k >> 1) + 1) >> 2; g < 4; ++g) lcg[g] = cb[g];
+                    // the reads stay here, a step ahead of their use (the
+                    // scheduler would sink them next to it)
+                    __builtin_amdgcn_sched_barrier(0);
                 }
                 lprev = lk;
             }
@@ -401,41 +411,45 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
             for (int v = 0; v < 16; ++v) acc[v] = (2 * v + (hi ? 1 : 0) == c) ? 1.f : 0.f;
             lds_cf4_p lcb = (lds_cf4_p)(Lc + (hi ? 16 : 0));
             asm volatile("" : "+v"(lcb));
-            f4 lq[4];
+            // column q's reads are issued two steps ahead, in slot q % 3
+            f4 lq[3][4];
 #pragma unroll
-            for (int g = 0; g < 4; ++g) lq[g] = lcb[g];
+            for (int g = 0; g < 4; ++g) {
+                lq[0][g] = lcb[g];
+                lq[1][g] = lcb[8 + g];
+            }
             float xprev = 0.f;
 #pragma unroll
             for (int q = 0; q < 32; ++q) {
                 const int hq = q & 1, vq = q >> 1;
-                // the next column's reads wait for x_{q-1}: without this
+                // the reads of column q+2 wait for x_{q-1}: without this
                 // order the scheduler hoists the reads of all 32 columns to
                 // the top, which exhausts the registers
-                f4 ln[4];
-                if (q < 31) {
+                if (q + 2 < 32) {
                     asm volatile("" : "+v"(lcb) : "v"(xprev));
 #pragma unroll
-                    for (int g = ((q + 1) >> 1) >> 2; g < 4; ++g) ln[g] = lcb[(q + 1) * 8 + g];
+                    for (int g = ((q + 2) >> 1) >> 2; g < 4; ++g)
+                        lq[(q + 2) % 3][g] = lcb[(q + 2) * 8 + g];
                 }
                 const float rq = rdlane(rv, q);
                 const float xq = acc[vq] * rq;                 // valid in half hq
                 acc[vq] = (hi == (hq == 1)) ? xq : acc[vq];
                 const float xb = bcast_half(xq, hq);
+                // whole groups live until here: a dead component's register
+                // reused early would wait for the read in flight into it
+#pragma unroll
+                for (int g = (q >> 1) >> 2; g < 4; ++g) asm volatile("" : "+v"(lq[q % 3][g]));
                 float Lq[16];
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
-                    Lq[4 * g] = lq[g].x; Lq[4 * g + 1] = lq[g].y;
-                    Lq[4 * g + 2] = lq[g].z; Lq[4 * g + 3] = lq[g].w;
+                    Lq[4 * g] = lq[q % 3][g].x; Lq[4 * g + 1] = lq[q % 3][g].y;
+                    Lq[4 * g + 2] = lq[q % 3][g].z; Lq[4 * g + 3] = lq[q % 3][g].w;
                 }
 #pragma unroll
                 for (int v = vq + 1; v < 16; ++v) acc[v] = fmaf(-Lq[v], xb, acc[v]);
                 if (hq == 0) {                                 // row q + 1 = 2 vq + 1: half 1
                     const float t = fmaf(-Lq[vq], xb, acc[vq]);
                     acc[vq] = hi ? t : acc[vq];
-                }
-                if (q < 31) {
-#pragma unroll
-                    for (int g = ((q + 1) >> 1) >> 2; g < 4; ++g) lq[g] = ln[g];
                 }
                 xprev = xb;
             }
