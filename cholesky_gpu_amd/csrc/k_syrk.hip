@@ -110,6 +110,29 @@ __device__ __forceinline__ v8h load8_f16(const operand_src& o, int64_t row, int 
     return __builtin_bit_cast(v8h, a);
 }
 
+// x = hi + mid + lo exactly, each term a bf16 (8 significant bits, taken by
+// truncation): the remainders x - hi and (x - hi) - mid are exact in FP32 and
+// the last one has at most 8 significant bits.  Eight values, packed as the
+// 16-bit matrix-core operand.
+__device__ __forceinline__ void split3_bf16(const float* f, v8bf& hi, v8bf& mid, v8bf& lo) {
+    uint32_t uh[8], um[8], ul[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        uh[e] = __float_as_uint(f[e]) & 0xffff0000u;
+        const float r1 = f[e] - __uint_as_float(uh[e]);
+        um[e] = __float_as_uint(r1) & 0xffff0000u;
+        const float r2 = r1 - __uint_as_float(um[e]);
+        ul[e] = __float_as_uint(r2) & 0xffff0000u;
+    }
+    uint4 wh, wm, wl;
+    wh.x = (uh[0] >> 16) | uh[1]; wh.y = (uh[2] >> 16) | uh[3]; wh.z = (uh[4] >> 16) | uh[5]; wh.w = (uh[6] >> 16) | uh[7];
+    wm.x = (um[0] >> 16) | um[1]; wm.y = (um[2] >> 16) | um[3]; wm.z = (um[4] >> 16) | um[5]; wm.w = (um[6] >> 16) | um[7];
+    wl.x = (ul[0] >> 16) | ul[1]; wl.y = (ul[2] >> 16) | ul[3]; wl.z = (ul[4] >> 16) | ul[5]; wl.w = (ul[6] >> 16) | ul[7];
+    hi = __builtin_bit_cast(v8bf, wh);
+    mid = __builtin_bit_cast(v8bf, wm);
+    lo = __builtin_bit_cast(v8bf, wl);
+}
+
 struct tile_ctx {
     float* C;
     int64_t ldc, m;
@@ -297,12 +320,42 @@ __device__ __forceinline__ void mm_16(const operand_src& oc, const operand_src& 
 }
 
 // NATIVE (runtime-K form only): both operands stored in FP32
-template <int QA, int KC, bool NATIVE>
+// SPLIT: 0 never, 1 always (the launch holds only such pairs), 2 per pair
+template <int QA, int KC, bool NATIVE, int SPLIT>
 __device__ __forceinline__ void mm_f32(const operand_src& oc, const operand_src& orow,
                                        const tile_ctx& t, int k, v16f acc[QA][QA]) {
     const int h = (threadIdx.x & 63) >> 5;
     int64_t ra[QA], rb[QA];
     operand_rows<QA>(t, ra, rb);
+    // FP32 against a format exact in bf16 (FP8 with its scale, or BF16): the
+    // FP32 operand as hi + mid + lo (exact), three bf16 products per 16-wide
+    // chunk; the products are exact and the sums FP32, as in the FP32
+    // matrix-core op, at a sixth of its cost per k
+    if (SPLIT == 1 || (SPLIT == 2 && (oc.prec == MPC_PREC_FP32) != (orow.prec == MPC_PREC_FP32) &&
+                       oc.prec != MPC_PREC_FP16 && orow.prec != MPC_PREC_FP16)) {
+        const bool col32 = oc.prec == MPC_PREC_FP32;
+        k_loop<0, 16>(k, [&](int kc) {
+            const int k0 = kc + 8 * h;
+            v8bf F[QA][3], Lw[QA];
+#pragma unroll
+            for (int q = 0; q < QA; ++q) {
+                float f[8];
+                load8_f32(col32 ? oc : orow, col32 ? ra[q] : rb[q], k0, f);
+                split3_bf16(f, F[q][0], F[q][1], F[q][2]);
+                Lw[q] = load8_bf16(col32 ? orow : oc, col32 ? rb[q] : ra[q], k0);
+            }
+#pragma unroll
+            for (int s3 = 0; s3 < 3; ++s3)
+#pragma unroll
+                for (int a = 0; a < QA; ++a)
+#pragma unroll
+                    for (int b = 0; b < QA; ++b)
+                        acc[a][b] = col32
+                            ? __builtin_amdgcn_mfma_f32_32x32x16_bf16(F[b][s3], Lw[a], acc[a][b], 0, 0, 0)
+                            : __builtin_amdgcn_mfma_f32_32x32x16_bf16(Lw[b], F[a][s3], acc[a][b], 0, 0, 0);
+        });
+        return;
+    }
     // k permutation: half h takes the 8-element groups k = 16c + 8h .. +8
     if constexpr (KC > 0) {
         // unrolled (small launches): branch-free loads for the all-FP32 pair
@@ -407,7 +460,7 @@ __device__ __forceinline__ void mm_f32(const operand_src& oc, const operand_src&
 // tile mixing classes across panels is updated once per class launch.
 // CLS < 0: every panel, the class chosen per workgroup and panel.
 template <int QA, int CLS, int FA, int FB, int KC>
-__global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_t m, int64_t ldc,
+__global__ __launch_bounds__(256, 2) void syrk_kernel(float* __restrict__ C, int64_t m, int64_t ldc,
                                                    int k, const mpc_syrk_panels P,
                                                    int64_t row_stride, int tile_rows,
                                                    const mpc_small* __restrict__ sm, int64_t npair,
@@ -464,8 +517,12 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
         const int fa = mpc_dev_is_e4m3(ocol.prec) ? 0 : 1, fb = mpc_dev_is_e4m3(orow.prec) ? 0 : 1;
         const bool f32pair = orow.prec == MPC_PREC_FP32 && ocol.prec == MPC_PREC_FP32;
         if (CLS >= 0) {            // this launch computes one class (and format pair) only
+            // FP32 class: FA 1 both FP32, 2 FP32 against a bf16-exact
+            // format (split products), 0 FP32 against FP16
+            const bool f16 = orow.prec == MPC_PREC_FP16 || ocol.prec == MPC_PREC_FP16;
+            const int f32kind = f32pair ? 1 : (f16 ? 0 : 2);
             if (cls != CLS || (CLS == MPC_PREC_E4M3 && (fa != FA || fb != FB)) ||
-                (CLS == MPC_PREC_FP32 && f32pair != (FA == 1)))
+                (CLS == MPC_PREC_FP32 && f32kind != FA))
                 continue;
         }
         any = true;
@@ -481,9 +538,11 @@ __global__ __launch_bounds__(256) void syrk_kernel(float* __restrict__ C, int64_
         } else if (c == MPC_PREC_FP16) {
             mm_16<QA, false, KC>(ocol, orow, t, k, acc);
         } else if (CLS == MPC_PREC_FP32) {
-            mm_f32<QA, KC, FA == 1>(ocol, orow, t, k, acc);
+            if constexpr (FA == 1) mm_f32<QA, KC, true, 0>(ocol, orow, t, k, acc);
+            else if constexpr (FA == 2) mm_f32<QA, KC, false, 1>(ocol, orow, t, k, acc);
+            else mm_f32<QA, KC, false, 0>(ocol, orow, t, k, acc);
         } else {                   // mixed-class launch: the K loop is unrolled (KC > 0)
-            mm_f32<QA, KC, false>(ocol, orow, t, k, acc);
+            mm_f32<QA, KC, false, 2>(ocol, orow, t, k, acc);
         }
     }
     if (!any) return;              // no panel of this launch's class on this tile
@@ -585,7 +644,9 @@ int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int
     if (class_mask & (1u << MPC_PREC_FP16)) MPC_SYRK_LAUNCH(2, MPC_PREC_FP16, 0, 0, 0);
     if (class_mask & (1u << MPC_PREC_FP32)) {
         MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 1, 0, 0);          // FP32 x FP32
-        if (class_mask & ~(1u << MPC_PREC_FP32))              // FP32 x a lower format
+        if (class_mask & ((1u << MPC_PREC_E4M3) | (1u << MPC_PREC_BF16)))
+            MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 2, 0, 0);      // FP32 x FP8 / BF16 (split)
+        if (class_mask & (1u << MPC_PREC_FP16))              // FP32 x FP16
             MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 0, 0, 0);
     }
     MPC_LAUNCH_CHECK();
