@@ -101,6 +101,7 @@ SIGNATURES = {
                              C.POINTER(D), C.POINTER(D), C.POINTER(D), C.POINTER(GpInfo)]),
     "mpc_quantize_panel": (I, [P, P, I64, I64, I64, I, P, P, I64, P]),
     "mpc_syrk": (I, [P, P, I64, I64, I64, P, I64, P, I, P]),
+    "mpc_syrk_masked": (I, [P, P, I64, I64, I64, P, I64, P, I, P, C.c_uint, C.c_uint]),
     "mpc_apply_precond": (I, [P, P, I64, I64, I, P, I64, P, I64]),
     "mpc_potrf_block": (I, [P, P, I, I64, C.POINTER(I)]),
     "mpc_convert_f8": (I, [P, P, P, I64, I]),

@@ -1439,6 +1439,25 @@ extern "C" int mpc_syrk(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_
                                    tile_rows, d_prec_of_tile, nullptr, cmask, fmask, 0, -1);
 }
 
+extern "C" int mpc_syrk_masked(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_t k,
+                               const uint8_t* d_payload, int64_t row_stride,
+                               const uint8_t* d_scales, int tile_rows,
+                               const uint8_t* d_prec_of_tile, unsigned class_mask,
+                               unsigned fp8_mask) {
+    MPC_CHECK(h && dC && d_payload && d_scales && d_prec_of_tile, "syrk: NULL argument");
+    MPC_CHECK(m >= 1 && ldc >= m && k >= 64 && k % 64 == 0, "syrk: bad shape (k multiple of 64)");
+    MPC_CHECK(tile_rows >= 128 && tile_rows % 128 == 0, "syrk: tile_rows multiple of 128");
+    MPC_CHECK(row_stride >= 4 * k, "syrk: row_stride must be >= 4*k");
+    const unsigned classes = (1u << MPC_PREC_FP32) | (1u << MPC_PREC_BF16) |
+                             (1u << MPC_PREC_FP16) | (1u << MPC_PREC_E4M3);
+    MPC_CHECK((class_mask & ~classes) == 0 && (fp8_mask & ~3u) == 0,
+              "syrk: class_mask 0x%x / fp8_mask 0x%x name no compute class", class_mask, fp8_mask);
+    MPC_HIP(hipSetDevice(h->device));
+    return mpc_launch_syrk_classes(h->stream, dC, m, ldc, (int)k, d_payload, row_stride, d_scales,
+                                   tile_rows, d_prec_of_tile, nullptr, class_mask, fp8_mask, 0,
+                                   -1);
+}
+
 extern "C" int mpc_apply_precond(mpc_handle* h, const float* dL, int64_t ldl, int64_t n,
                                  int nrhs, const double* dR, int64_t ldr, double* dZ,
                                  int64_t ldz) {

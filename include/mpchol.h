@@ -11,7 +11,8 @@
  *   mpc_precond_cg     <- precond_CG             src/mixed_cholesky.cu:76-78
  *   mpc_quantize_panel <- convert_fp32_to_mxe4m3 src/fp8_utils.cuh:27-66 and
  *                         convertFloatToHalf     src/kernels_macros.cuh:115-124
- *   mpc_syrk           <- LtMxfp8Matmul / LtHSgemm_fp16in_fp32out / LtSgemm
+ *   mpc_syrk,          <- LtMxfp8Matmul / LtHSgemm_fp16in_fp32out / LtSgemm
+ *   mpc_syrk_masked
  *                         src/fp8_utils.cuh:136,304,231 and the WMMA SYRK
  *                         src/syrk_gemm.cuh:14-124
  *   mpc_potrf_block    <- cusolverDnSpotrf call  src/switching_chol.cuh:86-95
@@ -252,6 +253,18 @@ int mpc_quantize_panel(mpc_handle* h, const float* dP, int64_t m, int64_t k, int
 int mpc_syrk(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_t k,
              const uint8_t* d_payload, int64_t row_stride, const uint8_t* d_scales,
              int tile_rows, const uint8_t* d_prec_of_tile);
+/* mpc_syrk (the same reference calls) with the tile formats present named by
+   the caller instead of read back from d_prec_of_tile: no device-to-host copy
+   and no synchronisation, so back-to-back calls queue only kernels.
+   class_mask: bit c set for every compute class c of a tile pair (the
+   higher-precision code of the pair's two formats, MPC_PREC_E4M3 for two 8-bit
+   formats); fp8_mask: bit 0 if an E4M3-element format (E4M3, MXE4M3) occurs,
+   bit 1 if an E5M2-element one does.  Tile pairs outside the masks are not
+   updated. */
+int mpc_syrk_masked(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_t k,
+                    const uint8_t* d_payload, int64_t row_stride, const uint8_t* d_scales,
+                    int tile_rows, const uint8_t* d_prec_of_tile, unsigned class_mask,
+                    unsigned fp8_mask);
 /* z = (L L^T)^{-1} r for nrhs columns (the preconditioner application of the
    PCG loop, the Dpotrs calls of mixed_cholesky.cu:147,166,204), FP64
    vectors, FP32 lower factor L from mpc_factor.  R is not modified. */

@@ -180,6 +180,36 @@ def test_syrk_ones_kat(handle, oracle):
         assert np.all(got[np.tril_indices(m)] == -float(k)), L.PREC_NAMES[prec]
 
 
+def test_syrk_masked_matches_syrk(handle):
+    """mpc_syrk_masked with the masks of the formats present gives mpc_syrk's
+    result bit for bit; a mask naming no compute class is refused."""
+    import ctypes as C
+    m, k, tr = 3 * 128 + 70, 256, 128
+    rng = np.random.default_rng(5)
+    P = rng.standard_normal((m, k)).astype(np.float32)
+    Cm = rng.standard_normal((m, m)).astype(np.float32)
+    cases = [([L.PREC_E4M3] * 4, 1 << L.PREC_E4M3, 1),
+             ([L.PREC_FP32, L.PREC_E4M3, L.PREC_E5M2, L.PREC_FP32],
+              (1 << L.PREC_FP32) | (1 << L.PREC_E4M3), 3)]
+    for precs, cmask, fmask in cases:
+        say("syrk_masked", [L.PREC_NAMES[p] for p in precs])
+        pay, sc, pt = S.quantize_panel(handle, to_dev_cm(P), precs, tr)
+        C1, C2 = to_dev_cm(Cm), to_dev_cm(Cm)
+        S.syrk(handle, C1, pay, sc, k, tr, pt)
+        rc = handle.lib.mpc_syrk_masked(handle.h, C.c_void_p(C2.data_ptr()), m, m, k,
+                                        C.c_void_p(pay.data_ptr()), pay.stride(0),
+                                        C.c_void_p(sc.data_ptr()), tr, C.c_void_p(pt.data_ptr()),
+                                        cmask, fmask)
+        assert rc == 0, rc
+        torch.cuda.synchronize()
+        assert torch.equal(C1, C2)
+    rc = handle.lib.mpc_syrk_masked(handle.h, C.c_void_p(C2.data_ptr()), m, m, k,
+                                    C.c_void_p(pay.data_ptr()), pay.stride(0),
+                                    C.c_void_p(sc.data_ptr()), tr, C.c_void_p(pt.data_ptr()),
+                                    1 << 7, 1)
+    assert rc == L.ERR_ARG
+
+
 # ---------------------------------------------------------------- generators
 def test_matgen_householder(handle, oracle):
     n = 1000
