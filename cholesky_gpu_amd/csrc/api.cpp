@@ -156,6 +156,7 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
                              &h->pcg, &h->scratch, &h->winv, &h->gp64, &h->gp32, &h->gpvec})
         ws_free(*w);
+    if (h->pcg_exec) hipGraphExecDestroy(h->pcg_exec);
     if (h->ev_col) hipEventDestroy(h->ev_col);
     if (h->ev_panel) hipEventDestroy(h->ev_panel);
     if (h->ev_wready) hipEventDestroy(h->ev_wready);
@@ -876,7 +877,7 @@ extern "C" void mpc_pcg_opts_default(mpc_pcg_opts* o) {
 
 namespace {
 struct pcg_bufs {
-    double *R, *Z, *P, *Q, *W, *Y, *part;
+    double *R, *Z, *P, *Q, *W, *Y, *X, *part;
     double *rz, *rz_new, *pq, *rr, *bb;
     int* active;
 };
@@ -970,13 +971,14 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
                          (size_t)mpc_block_prec_partial_doubles(n, mpc_block_bo(n), nrhs)),
         (size_t)nrhs * ((n + 2047) / 2048) + 64);
     const int hmax = o->maxit + 1;
-    const size_t nbytes = sizeof(double) * (6 * vec + part + 6 * (size_t)nrhs + 2 * (size_t)hmax + 64) +
+    const size_t nbytes = sizeof(double) * (7 * vec + part + 6 * (size_t)nrhs + 2 * (size_t)hmax + 64) +
                           sizeof(int) * 3 * (size_t)nrhs + 1024;
     if ((rc = mpc_ws_reserve(h->pcg, nbytes))) return rc;
     pcg_bufs b;
     double* base = (double*)h->pcg.ptr;
     b.R = base; b.Z = b.R + vec; b.P = b.Z + vec; b.Q = b.P + vec; b.W = b.Q + vec; b.Y = b.W + vec;
-    b.part = b.Y + vec;
+    b.X = b.Y + vec;
+    b.part = b.X + vec;
     b.rz = b.part + part; b.rz_new = b.rz + nrhs; b.pq = b.rz_new + nrhs; b.rr = b.pq + nrhs;
     b.bb = b.rr + nrhs;
     double* d_relres = b.bb + nrhs;
@@ -993,17 +995,22 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     hipLaunchKernelGGL(fill_int_kernel, dim3((nrhs + 255) / 256), dim3(256), 0, st, b.active, nrhs, 1);
     // B into a packed copy for the norms
     if ((rc = mpc_launch_dot(st, n, nrhs, dB, ldb, dB, ldb, b.bb, b.part))) return rc;
+    // The iterate lives in the workspace (b.X, packed, ld n) and is copied to
+    // dX at the end, so that the iteration graph below refers to the
+    // library's buffers only and is reused by the next solve in them.
     // x0
+    const double* cX = b.X;
+    const int64_t ldw = n;
     if (o->x0_mode == 0) {
-        double* tmpX = b.Z;    // M^-1 B, then copy to X
         if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.R, n))) return rc;
-        if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, tmpX, b))) return rc;
-        if ((rc = mpc_launch_copy2d(st, n, nrhs, tmpX, n, dX, ldx))) return rc;
+        if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.X, b))) return rc;   // M^-1 B
     } else if (o->x0_mode == 1) {
-        if ((rc = mpc_launch_axpby(st, n, nrhs, 0.0, dX, ldx, 0.0, dX, ldx))) return rc;
+        if ((rc = mpc_launch_axpby(st, n, nrhs, 0.0, b.X, ldw, 0.0, b.X, ldw))) return rc;
+    } else {
+        if ((rc = mpc_launch_copy2d(st, n, nrhs, dX, ldx, b.X, ldw))) return rc;
     }
     // R = B - A X
-    if ((rc = op_apply(st, op, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
+    if ((rc = op_apply(st, op, lda, n, nrhs, cX, ldw, b.Q, n, b.part))) return rc;
     if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.R, n))) return rc;
     if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.R, n))) return rc;
     if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return rc;
@@ -1038,7 +1045,7 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         int r;
         if ((r = op_apply(st, op, lda, n, nrhs, b.P, n, b.Q, n, b.part))) return r;
         if ((r = mpc_launch_dot(st, n, nrhs, b.P, n, b.Q, n, b.pq, b.part))) return r;
-        if ((r = mpc_launch_pcg_update_xr(st, n, nrhs, dX, ldx, b.R, n, b.P, n, b.Q, n, b.rz, b.pq,
+        if ((r = mpc_launch_pcg_update_xr(st, n, nrhs, b.X, ldw, b.R, n, b.P, n, b.Q, n, b.rz, b.pq,
                                           b.active)))
             return r;
         if ((r = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return r;
@@ -1050,32 +1057,54 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
                                      o->stop_rule, b.active, d_cand, d_iters, d_relres, d_ahist,
                                      d_bhist, hmax);
     };
-    // the iteration as a graph: one launch instead of ~60 (no memset nodes)
-    struct graph_holder {
-        hipGraphExec_t exec = nullptr;
-        ~graph_holder() { if (exec) (void)hipGraphExecDestroy(exec); }
-    } gh;
+    // the iteration as a graph: one launch instead of ~60 (no memset nodes).
+    // The instantiated graph is kept in the handle with every value the body
+    // passes to its kernels; a call with the same values (repeated solves in
+    // the same buffers) replays it without capturing again.
+    hipGraphExec_t exec = nullptr;
     if (o->graph && nactive > 0 && o->maxit > 0) {
-        hipGraph_t g = nullptr;
-        MPC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        const int brc = body();
-        const hipError_t ce = hipStreamEndCapture(st, &g);
-        if (brc) { if (g) (void)hipGraphDestroy(g); return brc; }
-        if (ce != hipSuccess) {
-            mpc_set_error("pcg: graph capture failed: %s", hipGetErrorString(ce));
-            return MPC_ERR_HIP;
+        const std::vector<uint64_t> key = {
+            (uint64_t)(uintptr_t)op.a32, (uint64_t)(uintptr_t)op.a64, (uint64_t)lda,
+            (uint64_t)n, (uint64_t)nrhs,
+            (uint64_t)(uintptr_t)dL, (uint64_t)ldl, (uint64_t)nb, (uint64_t)(uintptr_t)h->pcg.ptr,
+            (uint64_t)h->pcg.bytes, (uint64_t)hmax, __builtin_bit_cast(uint64_t, o->tol),
+            (uint64_t)o->stop_rule, (uint64_t)(uintptr_t)h->winv.ptr, (uint64_t)h->winv_bo};
+        if (!h->pcg_exec || h->pcg_key != key) {
+            if (h->pcg_exec) {
+                (void)hipGraphExecDestroy(h->pcg_exec);
+                h->pcg_exec = nullptr;
+                h->pcg_key.clear();
+            }
+            hipGraph_t g = nullptr;
+            MPC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const int brc = body();
+            const hipError_t ce = hipStreamEndCapture(st, &g);
+            if (brc) { if (g) (void)hipGraphDestroy(g); return brc; }
+            if (ce != hipSuccess) {
+                mpc_set_error("pcg: graph capture failed: %s", hipGetErrorString(ce));
+                return MPC_ERR_HIP;
+            }
+            const hipError_t ie = hipGraphInstantiate(&h->pcg_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) {
+                h->pcg_exec = nullptr;
+                mpc_set_error("pcg: graph instantiation failed: %s", hipGetErrorString(ie));
+                return MPC_ERR_HIP;
+            }
+            h->pcg_key = key;
         }
-        const hipError_t ie = hipGraphInstantiate(&gh.exec, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (ie != hipSuccess) {
-            gh.exec = nullptr;
-            mpc_set_error("pcg: graph instantiation failed: %s", hipGetErrorString(ie));
-            return MPC_ERR_HIP;
-        }
+        exec = h->pcg_exec;
     }
+    // The first iteration is launched eagerly: its kernels start at once,
+    // while the first graph launch (which the GPU would otherwise wait out,
+    // idle, after the host sync above) is prepared behind them.
+    bool first = true;
     auto run_body = [&]() -> int {
-        if (!gh.exec) return body();
-        MPC_HIP(hipGraphLaunch(gh.exec, st));
+        if (!exec || first) {
+            first = false;
+            return body();
+        }
+        MPC_HIP(hipGraphLaunch(exec, st));
         return 0;
     };
 
@@ -1098,7 +1127,7 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         if (any_cand) {
             if (o->stop_rule == 0) {
                 // confirm on the true residual B - A X; replace R where it lags
-                if ((rc = op_apply(st, op, lda, n, nrhs, dX, ldx, b.Q, n, b.part))) return rc;
+                if ((rc = op_apply(st, op, lda, n, nrhs, cX, ldw, b.Q, n, b.part))) return rc;
                 if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.W, n))) return rc;
                 if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.W, n))) return rc;
                 if ((rc = mpc_launch_dot(st, n, nrhs, b.W, n, b.W, n, b.rr, b.part))) return rc;
@@ -1143,6 +1172,7 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         MPC_HIP(hipMemcpyAsync(alphas.data(), d_ahist, sizeof(double) * m, hipMemcpyDeviceToHost, st));
         MPC_HIP(hipMemcpyAsync(betas.data(), d_bhist, sizeof(double) * m, hipMemcpyDeviceToHost, st));
     }
+    if ((rc = mpc_launch_copy2d(st, n, nrhs, b.X, ldw, dX, ldx))) return rc;
     MPC_HIP(hipEventRecord(e1, st));
     MPC_HIP(hipStreamSynchronize(st));
     float ms = 0;

@@ -59,6 +59,9 @@ struct mpc_workspace {
 };
 
 struct mpc_handle {
+    // the last PCG iteration graph and the arguments it was captured with
+    hipGraphExec_t        pcg_exec = nullptr;
+    std::vector<uint64_t> pcg_key;
     int          device = 0;
     hipStream_t  stream = nullptr;
     bool         own_stream = false;

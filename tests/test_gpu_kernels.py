@@ -402,6 +402,33 @@ def test_pcg_graph_replay_matches_eager(handle, oracle, check_every):
     assert np.array_equal(res[0][0], res[1][0])
 
 
+def test_pcg_cached_graph_matches_eager(handle, oracle):
+    """A second solve in the same buffers replays the iteration graph kept in
+    the handle (no new capture); with new right-hand sides in those buffers
+    it still gives bitwise the eager result, and a third solve with another
+    tolerance (a different graph) too."""
+    say("pcg cached graph")
+    n = 1536
+    A = hh(oracle, n, seed=78)
+    Ad = to_dev_cm(A)
+    Lf = Ad.clone()
+    rc, fi, _, _ = S.factor(handle, Lf, S.factor_opts(nb=256))
+    assert rc == 0
+    B = torch.empty((2, n), dtype=torch.float64, device="cuda")
+    X = torch.zeros_like(B)
+    for seed, tol in ((2, 1e-10), (3, 1e-10), (4, 1e-9)):
+        B.copy_(torch.from_numpy(np.random.default_rng(seed).standard_normal((2, n))))
+        out = []
+        for graph in (1, 0):
+            X.zero_()
+            _, pi, rc = S.pcg(handle, Ad, Lf, B, nb=256, X=X,
+                              opts=S.pcg_opts(tol=tol, maxit=200, graph=graph, check_every=0))
+            assert rc == 0, S.info_dict(pi)
+            out.append((X.cpu().numpy().copy(), pi.iters))
+        assert out[0][1] == out[1][1], (seed, out[0][1], out[1][1])
+        assert np.array_equal(out[0][0], out[1][0]), seed
+
+
 def test_factor_schedule_invariant(handle, oracle):
     """Per-panel updates (agg_panels = 1) give the same factor bit for bit
     under the look-ahead schedule (block column on its own stream in 64-row
