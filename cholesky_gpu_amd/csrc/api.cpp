@@ -1021,10 +1021,14 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     std::vector<double> hb(nrhs), hrr(nrhs), hrz(nrhs);
     std::vector<int> act(nrhs, 1), iters(nrhs, 0), cand(nrhs, 0);
     std::vector<double> relres(nrhs, 0.0), alphas, betas;
-    MPC_HIP(hipMemcpyAsync(hb.data(), b.bb, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-    MPC_HIP(hipMemcpyAsync(hrr.data(), b.rr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-    MPC_HIP(hipMemcpyAsync(hrz.data(), b.rz, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    // the scalars rz, rz_new, pq, rr, bb are contiguous (nrhs each): one copy
+    // (each copy is a separate transfer with its own latency)
+    std::vector<double> hsc(5 * (size_t)nrhs);
+    MPC_HIP(hipMemcpyAsync(hsc.data(), b.rz, sizeof(double) * 5 * nrhs, hipMemcpyDeviceToHost, st));
     MPC_HIP(hipStreamSynchronize(st));
+    std::copy(hsc.begin(), hsc.begin() + nrhs, hrz.begin());
+    std::copy(hsc.begin() + 3 * nrhs, hsc.begin() + 4 * nrhs, hrr.begin());
+    std::copy(hsc.begin() + 4 * nrhs, hsc.begin() + 5 * nrhs, hb.begin());
     std::vector<double> nb2(nrhs);
     int nactive = 0;
     for (int j = 0; j < nrhs; ++j) {
@@ -1034,9 +1038,18 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         relres[j] = res;
         if (res <= o->tol) act[j] = 0; else ++nactive;
     }
-    MPC_HIP(hipMemcpyAsync(b.active, act.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
-    MPC_HIP(hipMemcpyAsync(d_cand, cand.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
-    MPC_HIP(hipMemcpyAsync(d_iters, iters.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+    // the flags active, cand, iters are contiguous on the device: one copy
+    // each way (hflags is the host staging area of all three)
+    std::vector<int> hflags(3 * (size_t)nrhs);
+    auto flags_to_device = [&](int count) -> int {
+        std::copy(act.begin(), act.end(), hflags.begin());
+        std::copy(cand.begin(), cand.end(), hflags.begin() + nrhs);
+        std::copy(iters.begin(), iters.end(), hflags.begin() + 2 * nrhs);
+        MPC_HIP(hipMemcpyAsync(b.active, hflags.data(), sizeof(int) * count * nrhs,
+                               hipMemcpyHostToDevice, st));
+        return 0;
+    };
+    if ((rc = flags_to_device(3))) return rc;
 
     // One iteration for every active column; convergence is tested on the
     // device (candidates freeze), so iterations run back to back and the host
@@ -1117,11 +1130,13 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         for (int q = 0; q < nbody; ++q)
             if ((rc = run_body())) return rc;
         done += nbody;
-        MPC_HIP(hipMemcpyAsync(act.data(), b.active, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
-        MPC_HIP(hipMemcpyAsync(cand.data(), d_cand, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
-        MPC_HIP(hipMemcpyAsync(iters.data(), d_iters, sizeof(int) * nrhs, hipMemcpyDeviceToHost, st));
+        MPC_HIP(hipMemcpyAsync(hflags.data(), b.active, sizeof(int) * 3 * nrhs,
+                               hipMemcpyDeviceToHost, st));
         MPC_HIP(hipMemcpyAsync(relres.data(), d_relres, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
         MPC_HIP(hipStreamSynchronize(st));
+        std::copy(hflags.begin(), hflags.begin() + nrhs, act.begin());
+        std::copy(hflags.begin() + nrhs, hflags.begin() + 2 * nrhs, cand.begin());
+        std::copy(hflags.begin() + 2 * nrhs, hflags.begin() + 3 * nrhs, iters.begin());
         bool any_cand = false;
         for (int j = 0; j < nrhs; ++j) any_cand |= cand[j] != 0;
         if (any_cand) {
@@ -1147,8 +1162,7 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
                                            sizeof(double) * n, hipMemcpyDeviceToDevice, st));
                 }
             }
-            MPC_HIP(hipMemcpyAsync(b.active, act.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
-            MPC_HIP(hipMemcpyAsync(d_cand, cand.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, st));
+            if ((rc = flags_to_device(2))) return rc;        // active and cand
         }
         // next look: where the current convergence rate reaches tol (1..8)
         double res = 0.0;
@@ -1165,16 +1179,19 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         prev_res = res;
         prev_done = done;
     }
-    if (iters[0] > 0) {
-        const int m = std::min(iters[0], hmax);
-        alphas.resize(m);
-        betas.resize(m);
-        MPC_HIP(hipMemcpyAsync(alphas.data(), d_ahist, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-        MPC_HIP(hipMemcpyAsync(betas.data(), d_bhist, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-    }
     if ((rc = mpc_launch_copy2d(st, n, nrhs, b.X, ldw, dX, ldx))) return rc;
+    // the alpha and beta histories are contiguous (hmax each): one copy
+    const int mh = iters[0] > 0 ? std::min(iters[0], hmax) : 0;
+    std::vector<double> hab((size_t)hmax + mh);
+    if (mh > 0)
+        MPC_HIP(hipMemcpyAsync(hab.data(), d_ahist, sizeof(double) * (hmax + mh),
+                               hipMemcpyDeviceToHost, st));
     MPC_HIP(hipEventRecord(e1, st));
     MPC_HIP(hipStreamSynchronize(st));
+    if (mh > 0) {
+        alphas.assign(hab.begin(), hab.begin() + mh);
+        betas.assign(hab.begin() + hmax, hab.begin() + hmax + mh);
+    }
     float ms = 0;
     hipEventElapsedTime(&ms, e0, e1);
     hipEventDestroy(e0);
