@@ -750,23 +750,31 @@ static void flops_from_map(const uint8_t* map, int64_t n, int nb, mpc_factor_inf
         if (p == MPC_PREC_FP16 || q == MPC_PREC_FP16) return MPC_PREC_FP16;
         return MPC_PREC_E4M3;
     };
+    // Per step, the flops of a pair of row tiles (a, b) are 2 ra rb k (a > b)
+    // or ra (ra + 1) k (a = b), in the class of the pair's formats.  With
+    // S_p = sum of ra over the tiles of format p, the off-diagonal sum is
+    // sum_{p,q} S_p S_q - sum_a ra^2 in each class: linear in the tiles.  All
+    // terms are integers below 2^53, so the sums are exact in any order.
     for (int64_t j = 0; j < nt; ++j) {
         const int64_t e = std::min<int64_t>((j + 1) * nb, n), s = n - e;
         if (s <= 0) break;
         const int64_t ntl = (s + tr - 1) / tr;
-        std::vector<int> p(ntl);
+        double S[8] = {}, Q[8] = {}, D[8] = {};
         for (int64_t t = 0; t < ntl; ++t) {
-            p[t] = map[(j + 1 + t) + j * nt];
-            if (p[t] < 8) fi->tiles[p[t]]++;
+            const int p = map[(j + 1 + t) + j * nt];
+            if (p >= 8) continue;
+            fi->tiles[p]++;
+            const double r = (double)std::min<int64_t>(tr, s - t * tr);
+            S[p] += r;
+            Q[p] += r * r;
+            D[p] += r * (r + 1);
         }
         const double k = (double)nb;
-        for (int64_t a = 0; a < ntl; ++a) {
-            double ra = (double)std::min<int64_t>(tr, s - a * tr);
-            for (int64_t b = 0; b <= a; ++b) {
-                double rb = (double)std::min<int64_t>(tr, s - b * tr);
-                double f = a == b ? ra * (ra + 1) * k : 2.0 * ra * rb * k;
-                fi->syrk_flops[cls(p[a], p[b])] += f;
-            }
+        for (int p = 0; p < 8; ++p) {
+            if (S[p] == 0) continue;
+            for (int q = 0; q < 8; ++q)
+                if (S[q] != 0) fi->syrk_flops[cls(p, q)] += k * S[p] * S[q];
+            fi->syrk_flops[cls(p, p)] += k * (D[p] - Q[p]);
         }
     }
 }
@@ -839,8 +847,8 @@ static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     fi.flops_total = nd * nd * nd / 3.0 + nd * nd / 2.0 + nd / 6.0;
     if (rc >= 0) {
         std::vector<uint8_t> hm((size_t)nt * nt);
-        hipError_t e = hipMemcpy(hm.data(), map, hm.size(), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) { mpc_set_error("map copy: %s", hipGetErrorString(e)); return MPC_ERR_HIP; }
+        MPC_HIP(hipMemcpyAsync(hm.data(), map, hm.size(), hipMemcpyDeviceToHost, h->stream));
+        MPC_HIP(hipStreamSynchronize(h->stream));
         flops_from_map(hm.data(), n, nb, &fi);
     }
     if (info) *info = fi;
