@@ -474,8 +474,20 @@ void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* 
 // W (optional): 2 nI Bo^2 floats receiving the big-block inverses of L (and
 // their transposes) for the PCG sweeps, built on wstream as each big diagonal
 // block of L becomes final, overlapping the rest of the factor.
+// panels whose trailing updates beyond their group are aggregated: the
+// option, except in profile mode and unless precision tiles are exactly one
+// block (tile_rows == nb)
+static int agg_panels_of(const mpc_factor_opts* o) {
+    return (o->profile != 0 || tile_rows_for(o->nb) != o->nb) ? 1 : o->agg_panels;
+}
+
+// first_src (optional, aggregated schedule only): dA holds only the first
+// agg_panels block columns (the caller copied them); the rest of its lower
+// triangle is read from first_src (same leading dimension) by the first
+// aggregated update, which writes it to dA.
 static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const mpc_factor_opts* o,
-                       double eps, uint8_t* map, float* tracker, mpc_factor_info* fi, float* W) {
+                       double eps, uint8_t* map, float* tracker, mpc_factor_info* fi, float* W,
+                       const float* first_src = nullptr) {
     const int nb = o->nb;
     const int64_t nt = (n + nb - 1) / nb;
     const int tr = tile_rows_for(nb);
@@ -484,10 +496,13 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     if ((rc = wait_winv(h, st))) return rc;       // it reads dinv and the factor
     // panels live in a ring: 2 buffers, or 2 * agg_panels when the updates
     // beyond a group of agg_panels block columns are aggregated (the group
-    // being aggregated and the group being factored); aggregation needs
-    // precision tiles of exactly one block (tile_rows == nb)
+    // being aggregated and the group being factored)
     const bool prof = o->profile != 0;
-    const int AG = (prof || tr != nb) ? 1 : o->agg_panels;
+    const int AG = agg_panels_of(o);
+    if (first_src && (AG < 2 || (int64_t)AG * nb >= n)) {
+        mpc_set_error("factor: a deferred copy needs the aggregated schedule");
+        return MPC_ERR_ARG;
+    }
     const int nslot = 2 * AG;
     if ((rc = mpc_ws_reserve(h->panel, (size_t)nslot * (n + 128) * nb * 4))) return rc;
     if ((rc = mpc_ws_reserve(h->scales, (size_t)nslot * (n + 128) * (nb / 32)))) return rc;
@@ -504,8 +519,9 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
 
     hipLaunchKernelGGL(init_small_kernel, dim3(1), dim3(256), 0, st, sm);
     int64_t ninit = std::max<int64_t>(n, nt * nt);
+    // (the diagonal of A: from first_src when dA holds only the first columns)
     hipLaunchKernelGGL(init_factor_kernel, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st,
-                       dA, lda, n, tracker, map, nt);
+                       first_src ? first_src : dA, lda, n, tracker, map, nt);
     MPC_LAUNCH_CHECK();
 
     const int64_t pstride = (int64_t)(n + 128) * nb * 4;     // payload bytes per buffer
@@ -559,6 +575,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         P.p0 = (int)(p0 % nslot);
         P.np = np;
         P.nb = nb;
+        P.csrc = (first_src && p0 == 0) ? first_src + e + e * lda : nullptr;
         return mpc_launch_syrk_panels(us, dA + e + e * lda, n - e, lda, nb, P, (int64_t)nb * 4, tr,
                                       sm, t0, t1, qa, cmask, fmask);
     };
@@ -781,7 +798,8 @@ static void flops_from_map(const uint8_t* map, int64_t n, int nb, mpc_factor_inf
 
 static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const mpc_factor_opts* o,
                        uint8_t* d_map, float* d_tracker, mpc_factor_info* info,
-                       const float* restore_from, int64_t restore_ld, bool build_prec = false) {
+                       const float* restore_from, int64_t restore_ld, bool build_prec = false,
+                       const float* deferred_src = nullptr) {
     MPC_CHECK(h && dA, "factor: NULL argument");
     MPC_CHECK(n >= 1 && lda >= n, "factor: n=%lld lda=%lld", (long long)n, (long long)lda);
     mpc_factor_opts def;
@@ -828,7 +846,10 @@ static int factor_impl(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     mpc_factor_opts cur = *o;
     int attempt = 0, rc2_wait = 0;
     for (;;) {
-        rc = factor_once(h, dA, n, lda, &cur, eps, map, tracker, &fi, W);
+        // a deferred copy of A serves the first attempt only: a retry starts
+        // from A restored in full
+        rc = factor_once(h, dA, n, lda, &cur, eps, map, tracker, &fi, W,
+                         attempt == 0 ? deferred_src : nullptr);
         if (rc <= 0 || !o->fallback) break;
         // non-positive pivot: restore A, then retry with a smaller eps_prime
         // (twice), then with every update in FP32; an FP32 failure is final
@@ -1277,9 +1298,17 @@ extern "C" int mpc_precond_cg(mpc_handle* h, const float* dA, int64_t lda, int64
     if (rc) return rc;
     if ((rc = mpc_ws_reserve(h->amat, sizeof(float) * (size_t)n * n))) return rc;
     float* L = (float*)h->amat.ptr;
-    if ((rc = mpc_launch_copy_f32(h->stream, n, n, dA, lda, L, n))) return rc;
+    // The factor works on a copy of A.  With the aggregated schedule (and
+    // lda == n) only the first agg_panels block columns are copied here; the
+    // first aggregated update reads the rest from A itself and writes it to
+    // the copy, so that part of A is read once instead of copied first.
+    if ((rc = validate_factor_opts(fo))) return rc;
+    const int AG = agg_panels_of(fo);
+    const bool defer = lda == n && AG > 1 && (int64_t)AG * fo->nb < n;
+    if ((rc = mpc_launch_copy_f32(h->stream, n, defer ? (int64_t)AG * fo->nb : n, dA, lda, L, n)))
+        return rc;
     // the caller's A stays intact, so the fallback restores from it
-    rc = factor_impl(h, L, n, n, fo, nullptr, nullptr, finfo, dA, lda, true);
+    rc = factor_impl(h, L, n, n, fo, nullptr, nullptr, finfo, dA, lda, true, defer ? dA : nullptr);
     if (rc != 0) return rc;
     pcg_operator op;
     op.a32 = dA;

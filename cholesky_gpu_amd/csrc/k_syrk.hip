@@ -41,6 +41,30 @@ __device__ __forceinline__ int pair_class(int p, int q) {
     return MPC_PREC_E4M3;   // both 8-bit
 }
 
+// Position of a tile pair's launch in mpc_launch_syrk_panels' per-class
+// sequence (E4M3 element pairs 00, 01, 10, 11; BF16; FP16; FP32 against
+// FP32, against an FP8/BF16 format, against FP16).
+__device__ __forceinline__ int launch_rank_of(int cls, int fa, int fb) {
+    if (cls == MPC_PREC_E4M3) return 2 * fa + fb;
+    if (cls == MPC_PREC_BF16) return 4;
+    if (cls == MPC_PREC_FP16) return 5;
+    return fa == 1 ? 6 : (fa == 2 ? 7 : 8);      // FP32: fa is the partner kind
+}
+__device__ __forceinline__ bool is_e4m3_elem(int p) {
+    return p == MPC_PREC_E4M3 || p == MPC_PREC_MXE4M3;
+}
+__device__ __forceinline__ int launch_rank(int prow, int pcol) {
+    const int cls = pair_class(prow, pcol);
+    if (cls == MPC_PREC_E4M3)
+        return launch_rank_of(cls, is_e4m3_elem(pcol) ? 0 : 1, is_e4m3_elem(prow) ? 0 : 1);
+    if (cls == MPC_PREC_FP32) {
+        const bool f32pair = prow == MPC_PREC_FP32 && pcol == MPC_PREC_FP32;
+        const bool f16 = prow == MPC_PREC_FP16 || pcol == MPC_PREC_FP16;
+        return launch_rank_of(cls, f32pair ? 1 : (f16 ? 0 : 2), 0);
+    }
+    return launch_rank_of(cls, 0, 0);
+}
+
 struct operand_src {
     const uint8_t* payload;
     const uint8_t* scales;
@@ -135,6 +159,7 @@ __device__ __forceinline__ void split3_bf16(const float* f, v8bf& hi, v8bf& mid,
 
 struct tile_ctx {
     float* C;
+    const float* Cs;            // where C's old values are read (C, or a source copy)
     int64_t ldc, m;
     int64_t rowbase, colbase;   // first C row / col of this wave's quadrant
     bool diag;                  // this quadrant straddles the diagonal (rowbase == colbase)
@@ -160,6 +185,8 @@ __device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[QA][Q
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc(t.C + t.rowbase + t.colbase * t.ldc, 0, 0x7fffffff,
                                           0x00020000);
+    const __amdgpu_buffer_rsrc_t rsl = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(t.Cs) + t.rowbase + t.colbase * t.ldc, 0, 0x7fffffff, 0x00020000);
     const int ld4 = (int)t.ldc * 4;
     const int rlim = (int)(t.m - t.rowbase), clim = (int)(t.m - t.colbase);
     int loff[QA];
@@ -180,7 +207,7 @@ __device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[QA][Q
 #pragma unroll
                 for (int i = 0; i < 16; ++i)
                     v[a][b][i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                        rs, loff[a], (32 * b + 8 * (i >> 2) + (i & 3)) * ld4, 0));
+                        rsl, loff[a], (32 * b + 8 * (i >> 2) + (i & 3)) * ld4, 0));
     } else {
 #pragma unroll
         for (int a = 0; a < QA; ++a)
@@ -191,7 +218,7 @@ __device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[QA][Q
                     const int r = 32 * a + n, c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
                     const int rc = r < rlim ? r : rlim - 1, cc = c < clim ? c : clim - 1;
                     v[a][b][i] = __uint_as_float(
-                        __builtin_amdgcn_raw_buffer_load_b32(rs, rc * 4 + cc * ld4, 0, 0));
+                        __builtin_amdgcn_raw_buffer_load_b32(rsl, rc * 4 + cc * ld4, 0, 0));
                 }
     }
 #pragma unroll
@@ -489,6 +516,7 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(float* __restrict__ C, int
 
     tile_ctx t;
     t.C = C;
+    t.Cs = C;
     t.ldc = ldc;
     t.m = m;
     t.rowbase = ti * TILE + 32 * QA * wr;
@@ -502,6 +530,24 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(float* __restrict__ C, int
 #pragma unroll
         for (int b2 = 0; b2 < QA; ++b2) acc[a][b2] = v16f{};
     bool any = false;
+    if (P.csrc) {
+        // C's old values come from csrc in the first launch, in the host's
+        // launch order, whose class covers one of this tile's panels
+        int first = 0;
+        if (CLS >= 0) {
+            int lo = 99;
+            for (int q = 0; q < P.np; ++q) {
+                const int slot = (P.p0 + q) % P.slots;
+                const uint8_t* prec = P.prec + slot * 1024 + (P.rows0 - (int64_t)q * P.nb) / tile_rows;
+                const int r = launch_rank(prec[trow], prec[tcol]);
+                lo = r < lo ? r : lo;
+            }
+            first = lo == launch_rank_of(CLS, FA, FB);
+        } else {
+            first = 1;                 // one launch for every class
+        }
+        if (first) t.Cs = P.csrc;
+    }
     for (int q = 0; q < P.np; ++q) {
         const int slot = (P.p0 + q) % P.slots;
         const int64_t roff = P.rows0 - (int64_t)q * P.nb;   // C row 0 within this panel

@@ -177,6 +177,10 @@ struct mpc_syrk_panels {
     int64_t pstride, sstride;    // bytes between slots (prec: 1024 per slot)
     int64_t rows0;
     int slots, p0, np, nb;
+    // optional: C's values are read from csrc (same leading dimension) by
+    // the first launch of the class sequence that touches a tile, and written
+    // to C; later launches read C.  nullptr: C in place.
+    const float* csrc;
 };
 // C(lower) -= sum_p Q_p Q_p^T over tile columns [tcol0, tcol1), tiles of
 // 64 * qa rows: qa = 1 one launch with the class chosen per workgroup and

@@ -429,6 +429,38 @@ def test_pcg_cached_graph_matches_eager(handle, oracle):
         assert np.array_equal(out[0][0], out[1][0]), seed
 
 
+@pytest.mark.parametrize("agg,ladder", [
+    (2, (L.PREC_E4M3, L.PREC_FP32)),
+    (3, (L.PREC_E5M2, L.PREC_MXE4M3, L.PREC_BF16, L.PREC_FP32)),
+])
+def test_precond_cg_deferred_copy(handle, oracle, agg, ladder):
+    """precond_cg copies only the first agg_panels block columns of A and lets
+    the first aggregated update read the rest from A: the solution and the
+    factor statistics are bitwise those of factoring a full copy and then
+    running the PCG on it."""
+    n = 2304
+    say("precond_cg deferred copy", agg, [L.PREC_NAMES[p] for p in ladder])
+    A = hh(oracle, n, seed=91)
+    Ad = to_dev_cm(A)
+    B = torch.from_numpy(np.random.default_rng(9).standard_normal((2, n))).cuda()
+    fo = S.factor_opts(nb=256, update_prec=L.PREC_SWITCH, ladder=ladder, eps_prime=0.25,
+                       fallback=0, agg_panels=agg)
+    po = S.pcg_opts(tol=1e-10, maxit=300)
+    X1, fi1, pi1, rc1 = S.precond_cg(handle, Ad, B, fo, po)
+    assert rc1 == 0, S.info_dict(pi1)
+    Lf = Ad.clone()
+    rc, fi2, _, _ = S.factor(handle, Lf, fo)
+    assert rc == 0
+    X2, pi2, rc2 = S.pcg(handle, Ad, Lf, B, nb=256, opts=po)
+    assert rc2 == 0
+    assert list(fi1.syrk_flops) == list(fi2.syrk_flops)
+    assert sum(fi1.syrk_flops[p] for p in range(8) if p != L.PREC_FP32) > 0   # mixed classes
+    assert fi1.syrk_flops[L.PREC_FP32] > 0
+    assert pi1.iters == pi2.iters
+    assert torch.equal(X1, X2)
+    assert torch.equal(Ad, to_dev_cm(A))            # A itself untouched
+
+
 def test_factor_schedule_invariant(handle, oracle):
     """Per-panel updates (agg_panels = 1) give the same factor bit for bit
     under the look-ahead schedule (block column on its own stream in 64-row
