@@ -14,7 +14,9 @@
 //              E8M0 scales applied by the matrix core: MX per 32, or the tile
 //              scale replicated per 32)
 //   16-bit     v_mfma_f32_32x32x16_bf16 / _f16 (FP8 operands converted exactly)
-//   FP32       v_mfma_f32_32x32x2f32 (lower formats expanded exactly)
+//   FP32       v_mfma_f32_32x32x2f32 for FP32 x FP32 and FP32 x FP16 pairs;
+//              FP32 x FP8/BF16 pairs as three exact bf16 products per chunk
+//              (the FP32 operand split as hi + mid + lo)
 //
 // Launch: one workgroup (4 waves) per 128x128 lower tile of C, wave (wr, wc)
 // owns a 64x64 quadrant as 2x2 MFMA blocks (64x64 tiles of 1x1 blocks for the
