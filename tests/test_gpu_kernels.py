@@ -292,8 +292,11 @@ def test_factor_matches_oracle(handle, oracle, n, nb, prec, ladder, agg):
     assert flips <= max(1, low.sum() // 50), (flips, low.sum())
     # Schur-diagonal tracker
     assert np.allclose(dg.cpu().numpy(), udo, rtol=1e-3, atol=1e-3 * np.abs(np.diag(A)).max())
-    # flop accounting
+    # flop accounting, per compute class (same schedule: the oracle ran on the GPU's map)
     assert abs(sum(fi.syrk_flops) - sum(ofi.flops)) / sum(ofi.flops) < 1e-9
+    for c in range(8):
+        assert abs(fi.syrk_flops[c] - ofi.flops[c]) <= 1e-9 * sum(ofi.flops), (c, list(fi.syrk_flops),
+                                                                              list(ofi.flops))
 
 
 @pytest.mark.parametrize("n,nb,ladder,nrhs", [
