@@ -1137,15 +1137,8 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         }
         exec = h->pcg_exec;
     }
-    // The first iteration is launched eagerly: its kernels start at once,
-    // while the first graph launch (which the GPU would otherwise wait out,
-    // idle, after the host sync above) is prepared behind them.
-    bool first = true;
     auto run_body = [&]() -> int {
-        if (!exec || first) {
-            first = false;
-            return body();
-        }
+        if (!exec) return body();
         MPC_HIP(hipGraphLaunch(exec, st));
         return 0;
     };
