@@ -23,10 +23,11 @@ constexpr int NRC = 8;           // right-hand sides per pass
 // (A_IJ x_J) and column sums (A_IJ^T x_I) go to fixed partial slots
 //   P[K][L][r]:  L >= K row-sum partial of tile (K, L); L < K column-sum
 //               partial of tile (L, K),
-// and symv_reduce sums P[K][0..nt) in order (deterministic).  Tile edge per
-// matrix element type: 128 for FP32 (65 KB of LDS), 64 for FP64 (33 KB).
-template <typename T> struct symv_tile { static constexpr int ST = 128; };
-template <> struct symv_tile<double> { static constexpr int ST = 64; };
+// and symv_reduce sums P[K][0..nt) in order (deterministic).  Tile edge 64:
+// 17 KB (FP32) or 33 KB (FP64) of LDS, so several tiles share a CU and one
+// tile's loads overlap another's sums.  (A 128 edge for FP32, 65 KB and two
+// workgroups per CU, was slower.)
+constexpr int SYMV_ST = 64;
 template <typename T> struct vec16 { using type = float4; };
 template <> struct vec16<double> { using type = double2; };
 
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(256) void symv_tile_kernel(const T* __restrict__ A,
                                                        int64_t n, int nt, int nrhs,
                                                        const double* __restrict__ X, int64_t ldx,
                                                        double* __restrict__ P) {
-    constexpr int ST = symv_tile<T>::ST;
+    constexpr int ST = SYMV_ST;
     __shared__ T As[ST * (ST + 1)];
     __shared__ double xi[ST], xj[ST];
     // tile index -> (I, J), I <= J, row-major over the upper triangle
@@ -260,15 +261,14 @@ inline unsigned blocks_for(int64_t n) { return (unsigned)((n + RB - 1) / RB); }
 }  // namespace
 
 int64_t mpc_symv_partial_doubles(int64_t n, int nrhs) {
-    // the larger of the FP32 (tile 128) and FP64 (tile 64) layouts
-    const int64_t n128 = (n + 127) / 128, n64 = (n + 63) / 64;
-    return std::max(n128 * n128 * 128, n64 * n64 * 64) * nrhs;
+    const int64_t nt = (n + SYMV_ST - 1) / SYMV_ST;
+    return nt * nt * SYMV_ST * nrhs;
 }
 
 template <typename T>
 static int launch_symv_t(hipStream_t st, const T* A, int64_t lda, int64_t n, int nrhs,
                          const double* X, int64_t ldx, double* Y, int64_t ldy, double* part) {
-    constexpr int ST = symv_tile<T>::ST;
+    constexpr int ST = SYMV_ST;
     const int64_t nt = (n + ST - 1) / ST;
     const int64_t ntile = nt * (nt + 1) / 2;
     hipLaunchKernelGGL(symv_tile_kernel<T>, dim3((unsigned)ntile), dim3(256), 0, st, A, lda, n,
