@@ -107,28 +107,36 @@ __global__ __launch_bounds__(256) void symv_tile_kernel(const T* __restrict__ A,
     }
 }
 
+// 64 rows per workgroup (lane = row).  Wave w sums the partials of tiles
+// [w q, (w+1) q), q = ceil(nt / 4), in batches of 16 independent loads
+// (clamped tile index, zero weight past the end) in tile order; the four
+// quarter sums are then added in order.
 template <int ST>
 __global__ __launch_bounds__(256) void symv_reduce_kernel(int64_t n, int nt, int nrhs,
                                                          const double* __restrict__ P,
                                                          double* __restrict__ Y, int64_t ldy) {
-    const int64_t row = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t row0 = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t row = row0 < n ? row0 : n - 1;
     const int j = blockIdx.y;
-    if (row >= n) return;
     const int64_t K = row / ST, r = row % ST;
-    // batches of 16 independent loads (clamped tile index, zero weight past
-    // the end), added in tile order
+    const int q = (nt + 3) / 4, l0 = w * q, l1 = l0 + q < nt ? l0 + q : nt;
     double s = 0.0;
-    for (int L0 = 0; L0 < nt; L0 += 16) {
+    for (int L0 = l0; L0 < l1; L0 += 16) {
         double v[16];
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
-            const int64_t L = L0 + u < nt ? L0 + u : nt - 1;
+            const int64_t L = L0 + u < l1 ? L0 + u : l1 - 1;
             v[u] = P[((K * nt + L) * nrhs + j) * ST + r];
         }
 #pragma unroll
-        for (int u = 0; u < 16; ++u) s += L0 + u < nt ? v[u] : 0.0;
+        for (int u = 0; u < 16; ++u) s += L0 + u < l1 ? v[u] : 0.0;
     }
-    Y[j * ldy + row] = s;
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && row0 < n)
+        Y[j * ldy + row0] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 // ---------------------------------------------------------------- dots and updates
@@ -273,7 +281,8 @@ static int launch_symv_t(hipStream_t st, const T* A, int64_t lda, int64_t n, int
     const int64_t ntile = nt * (nt + 1) / 2;
     hipLaunchKernelGGL(symv_tile_kernel<T>, dim3((unsigned)ntile), dim3(256), 0, st, A, lda, n,
                        (int)nt, nrhs, X, ldx, part);
-    hipLaunchKernelGGL(symv_reduce_kernel<ST>, dim3(blocks_for(n), (unsigned)nrhs), dim3(256), 0, st,
+    hipLaunchKernelGGL(symv_reduce_kernel<ST>, dim3((unsigned)((n + 63) / 64), (unsigned)nrhs),
+                       dim3(256), 0, st,
                        n, (int)nt, nrhs, part, Y, ldy);
     MPC_LAUNCH_CHECK();
     return 0;

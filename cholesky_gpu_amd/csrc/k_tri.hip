@@ -389,29 +389,36 @@ __global__ __launch_bounds__(64) void tri_gemv_tile_kernel(const float* __restri
     }
 }
 
+// 64 rows per workgroup (lane = row, one TG chunk row).  Wave w sums the
+// chunks jlo + w, jlo + w + 4, ... in batches of 8 independent loads
+// (clamped chunk index, zero weight beyond jhi): one batch when the block
+// has at most 32 chunks.  The four wave sums are then added in order.
 __global__ __launch_bounds__(256) void tri_gemv_reduce_kernel(const double* __restrict__ part,
                                                              int bI, int nrhs, int upper,
                                                              double* __restrict__ out,
                                                              int64_t ldo) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i0 = blockIdx.x * 64 + lane;
+    const int i = i0 < bI ? i0 : bI - 1;
     const int r = blockIdx.y;
-    if (i >= bI) return;
     const int nb = (bI + TG - 1) / TG, ib = i / TG;
     const int jlo = upper ? ib : 0, jhi = upper ? nb - 1 : ib;
-    // batches of 8 independent loads (clamped chunk index, zero weight beyond
-    // jhi), added in chunk order
     double s = 0.0;
-    for (int j0 = jlo; j0 <= jhi; j0 += 8) {
+    for (int j0 = jlo + w; j0 <= jhi; j0 += 32) {
         double v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int jb = j0 + u <= jhi ? j0 + u : jhi;
+            const int jb = j0 + 4 * u <= jhi ? j0 + 4 * u : jhi;
             v[u] = part[((int64_t)jb * nrhs + r) * bI + i];
         }
 #pragma unroll
-        for (int u = 0; u < 8; ++u) s += j0 + u <= jhi ? v[u] : 0.0;
+        for (int u = 0; u < 8; ++u) s += j0 + 4 * u <= jhi ? v[u] : 0.0;
     }
-    out[i + (int64_t)r * ldo] = s;
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && i0 < bI)
+        out[i0 + (int64_t)r * ldo] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
 }
 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
@@ -529,7 +536,7 @@ static int tri_gemv(hipStream_t st, const float* M, int Bo, int bI, int upper, c
     else
         hipLaunchKernelGGL(tri_gemv_tile_kernel<false>, grid, dim3(64), 0, st, M, Bo, bI, V, ldv,
                            nrhs, pin, S, part);
-    hipLaunchKernelGGL(tri_gemv_reduce_kernel, dim3(cdiv(bI, 256), (unsigned)nrhs), dim3(256), 0, st,
+    hipLaunchKernelGGL(tri_gemv_reduce_kernel, dim3(cdiv(bI, 64), (unsigned)nrhs), dim3(256), 0, st,
                        part, bI, nrhs, upper, out, ldo);
     MPC_LAUNCH_CHECK();
     return 0;
