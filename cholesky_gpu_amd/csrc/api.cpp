@@ -998,7 +998,7 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     const size_t part = std::max<size_t>(
         std::max<size_t>((size_t)mpc_symv_partial_doubles(n, nrhs),
                          (size_t)mpc_block_prec_partial_doubles(n, mpc_block_bo(n), nrhs)),
-        (size_t)nrhs * ((n + 2047) / 2048) + 64);
+        2 * (size_t)nrhs * ((n + 2047) / 2048) + 64);      // (two dots at once)
     const int hmax = o->maxit + 1;
     const size_t nbytes = sizeof(double) * (7 * vec + part + 6 * (size_t)nrhs + 2 * (size_t)hmax + 64) +
                           sizeof(int) * 3 * (size_t)nrhs + 1024;
@@ -1044,8 +1044,7 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.R, n))) return rc;
     if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return rc;
     if ((rc = mpc_launch_copy2d(st, n, nrhs, b.Z, n, b.P, n))) return rc;
-    if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.Z, n, b.rz, b.part))) return rc;
-    if ((rc = mpc_launch_dot(st, n, nrhs, b.R, n, b.R, n, b.rr, b.part))) return rc;
+    if ((rc = mpc_launch_dot2(st, n, nrhs, b.R, n, b.Z, n, b.rz, b.R, n, b.rr, b.part))) return rc;
 
     std::vector<double> hb(nrhs), hrr(nrhs), hrz(nrhs);
     std::vector<int> act(nrhs, 1), iters(nrhs, 0), cand(nrhs, 0);
@@ -1091,8 +1090,8 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
                                           b.active)))
             return r;
         if ((r = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return r;
-        if ((r = mpc_launch_dot(st, n, nrhs, b.R, n, b.Z, n, b.rz_new, b.part))) return r;
-        if ((r = mpc_launch_dot(st, n, nrhs, b.R, n, b.R, n, b.rr, b.part))) return r;
+        if ((r = mpc_launch_dot2(st, n, nrhs, b.R, n, b.Z, n, b.rz_new, b.R, n, b.rr, b.part)))
+            return r;
         if ((r = mpc_launch_pcg_update_p(st, n, nrhs, b.P, n, b.Z, n, b.rz_new, b.rz, b.active)))
             return r;
         return mpc_launch_pcg_status(st, nrhs, b.rz, b.rz_new, b.pq, b.rr, b.bb, o->tol,

@@ -140,11 +140,16 @@ __global__ __launch_bounds__(256) void symv_reduce_kernel(int64_t n, int nt, int
 }
 
 // ---------------------------------------------------------------- dots and updates
+// blockIdx.z selects the second operand: 0 -> b, 1 -> b2 (two dots with the
+// same first operand in one launch); partials of z at part + z nrhs nparts
 __global__ __launch_bounds__(RB) void dot_partial_kernel(int64_t n, const double* __restrict__ a,
-                                                        int64_t lda_, const double* __restrict__ b,
-                                                        int64_t ldb, double* __restrict__ part) {
+                                                        int64_t lda_, const double* __restrict__ b0,
+                                                        int64_t ldb0, const double* __restrict__ b2,
+                                                        int64_t ldb2, double* __restrict__ part) {
     __shared__ double red[RB];
     const int j = blockIdx.y;
+    const double* __restrict__ b = blockIdx.z ? b2 : b0;
+    const int64_t ldb = blockIdx.z ? ldb2 : ldb0;
     const int64_t r0 = (int64_t)blockIdx.x * RB * 8;
     double s = 0.0;
     for (int q = 0; q < 8; ++q) {
@@ -157,13 +162,18 @@ __global__ __launch_bounds__(RB) void dot_partial_kernel(int64_t n, const double
         if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[(int64_t)j * gridDim.x + blockIdx.x] = red[0];
+    if (threadIdx.x == 0)
+        part[((int64_t)blockIdx.z * gridDim.y + j) * gridDim.x + blockIdx.x] = red[0];
 }
 
+// blockIdx.y selects the dot: 0 -> out, 1 -> out2
 __global__ __launch_bounds__(RB) void dot_final_kernel(int nparts, const double* __restrict__ part,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out0,
+                                                      double* __restrict__ out2) {
     __shared__ double red[RB];
     const int j = blockIdx.x;
+    double* __restrict__ out = blockIdx.y ? out2 : out0;
+    part += (int64_t)blockIdx.y * gridDim.x * nparts;
     double s = 0.0;
     for (int q = threadIdx.x; q < nparts; q += RB) s += part[(int64_t)j * nparts + q];
     red[threadIdx.x] = s;
@@ -301,11 +311,24 @@ int mpc_launch_symv64(hipStream_t st, const double* A, int64_t lda, int64_t n, i
 int mpc_launch_dot(hipStream_t st, int64_t n, int nrhs, const double* a, int64_t lda_,
                    const double* b, int64_t ldb, double* out, double* part) {
     const unsigned nparts = (unsigned)((n + RB * 8 - 1) / (RB * 8));
-    hipLaunchKernelGGL(dot_partial_kernel, dim3(nparts, (unsigned)nrhs), dim3(RB), 0, st, n, a,
-                       lda_, b, ldb, part);
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nparts, (unsigned)nrhs, 1), dim3(RB), 0, st, n, a,
+                       lda_, b, ldb, b, ldb, part);
     MPC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(dot_final_kernel, dim3((unsigned)nrhs), dim3(RB), 0, st, (int)nparts, part,
-                       out);
+    hipLaunchKernelGGL(dot_final_kernel, dim3((unsigned)nrhs, 1), dim3(RB), 0, st, (int)nparts, part,
+                       out, out);
+    MPC_LAUNCH_CHECK();
+    return 0;
+}
+
+int mpc_launch_dot2(hipStream_t st, int64_t n, int nrhs, const double* a, int64_t lda_,
+                    const double* b, int64_t ldb, double* out_b, const double* c, int64_t ldc,
+                    double* out_c, double* part) {
+    const unsigned nparts = (unsigned)((n + RB * 8 - 1) / (RB * 8));
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(nparts, (unsigned)nrhs, 2), dim3(RB), 0, st, n, a,
+                       lda_, b, ldb, c, ldc, part);
+    MPC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(dot_final_kernel, dim3((unsigned)nrhs, 2), dim3(RB), 0, st, (int)nparts, part,
+                       out_b, out_c);
     MPC_LAUNCH_CHECK();
     return 0;
 }

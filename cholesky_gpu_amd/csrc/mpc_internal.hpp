@@ -205,6 +205,11 @@ int mpc_launch_symv64(hipStream_t st, const double* A, int64_t lda, int64_t n, i
 int64_t mpc_symv_partial_doubles(int64_t n, int nrhs);
 int mpc_launch_dot(hipStream_t st, int64_t n, int nrhs, const double* a, int64_t lda_,
                    const double* b, int64_t ldb, double* out, double* part);
+// out_b = a . b and out_c = a . c per column in two launches (part: twice
+// the partials of one dot); each sum in the same order as mpc_launch_dot's
+int mpc_launch_dot2(hipStream_t st, int64_t n, int nrhs, const double* a, int64_t lda_,
+                    const double* b, int64_t ldb, double* out_b, const double* c, int64_t ldc,
+                    double* out_c, double* part);
 int mpc_launch_pcg_update_xr(hipStream_t st, int64_t n, int nrhs, double* X, int64_t ldx,
                              double* R, int64_t ldr, const double* P, int64_t ldp,
                              const double* Q, int64_t ldq, const double* rz, const double* pq,
