@@ -169,7 +169,10 @@ __global__ __launch_bounds__(256) void transpose_blocks_kernel(const float* __re
 // later block.
 constexpr int NR = 4;       // right-hand sides per pass
 constexpr int TG = 64;      // tile edge of the triangular GEMV
-constexpr int FW_CW = 256;  // forward: columns per split (4 waves x 64)
+// forward: columns per split (4 waves x FW_CPW).  At n = 16384 (Bo = 2048)
+// 128 beat 64, 256 and 512: more workgroups with fewer loads in flight each.
+constexpr int FW_CW = 128;
+constexpr int FW_CPW = FW_CW / 4;
 constexpr int BW_CPW = 8;   // backward: columns per wave (32 per workgroup)
 constexpr int BW_RW = 512;  // backward: rows per split (8 per lane)
 
@@ -183,7 +186,7 @@ __device__ __forceinline__ double rdlane_d(double v, int l) {
 // Forward, big block I = rows [p0, p0 + bI):
 //   part[(s nrhs + j) bI + i] = sum_{c in split s} L[p0 + i][c] Y[c][j],  c < p0
 // 64 rows per workgroup (lane = row: each column load is a 256-byte
-// segment), split s = FW_CW columns, wave w its 64 of them, all 64 loads of
+// segment), split s = FW_CW columns, wave w its FW_CPW of them, all loads of
 // a lane in flight before the first product; the split's Y values are staged
 // in LDS and read back by all lanes at once.
 __global__ __launch_bounds__(256) void fwd_gemv_partial_kernel(
@@ -199,10 +202,10 @@ __global__ __launch_bounds__(256) void fwd_gemv_partial_kernel(
     const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
     const int64_t cs = (int64_t)sp * FW_CW;
     const float* Lr = L + (p0 + ic);
-    float l[64];
+    float l[FW_CPW];
 #pragma unroll
-    for (int u = 0; u < 64; ++u) {
-        const int64_t c = cs + wave * 64 + u;
+    for (int u = 0; u < FW_CPW; ++u) {
+        const int64_t c = cs + wave * FW_CPW + u;
         const float x = Lr[(c < p0 ? c : p0 - 1) * ldl];     // clamped, then masked
         l[u] = c < p0 ? x : 0.f;
     }
@@ -221,9 +224,9 @@ __global__ __launch_bounds__(256) void fwd_gemv_partial_kernel(
         double a0 = 0.0, a1 = 0.0;
         if (j < nj) {
 #pragma unroll
-            for (int u = 0; u < 64; u += 2) {
-                a0 = fma((double)l[u], ys[j][wave * 64 + u], a0);
-                a1 = fma((double)l[u + 1], ys[j][wave * 64 + u + 1], a1);
+            for (int u = 0; u < FW_CPW; u += 2) {
+                a0 = fma((double)l[u], ys[j][wave * FW_CPW + u], a0);
+                a1 = fma((double)l[u + 1], ys[j][wave * FW_CPW + u + 1], a1);
             }
         }
         acc[j] = a0 + a1;
