@@ -174,7 +174,8 @@ constexpr int TG = 64;      // tile edge of the triangular GEMV
 constexpr int FW_CW = 128;
 constexpr int FW_CPW = FW_CW / 4;
 constexpr int BW_CPW = 8;   // backward: columns per wave (32 per workgroup)
-constexpr int BW_RW = 512;  // backward: rows per split (8 per lane)
+constexpr int BW_RW = 512;  // backward: rows per split (BW_RPL per lane)
+constexpr int BW_RPL = BW_RW / 64;
 
 __device__ __forceinline__ double rdlane_d(double v, int l) {
     const uint64_t b = __builtin_bit_cast(uint64_t, v);
@@ -246,8 +247,8 @@ __global__ __launch_bounds__(256) void fwd_gemv_partial_kernel(
 // Backward, big block I = columns [p0, p0 + bI), rows below it [p1, n):
 //   part[(s nrhs + j) bI + c] = sum_{r in split s} L[r][p0 + c] X[r][j]
 // 32 columns per workgroup (8 per wave), split s = BW_RW rows; lane k of a
-// wave holds rows lane + 64k of each of its columns (coalesced 256-byte
-// loads, 64 in flight), then the 64 lane sums of a column are added through
+// wave holds rows lane + 64k, k < BW_RPL, of each of its columns (coalesced
+// 256-byte loads, 8 BW_RPL in flight), then the 64 lane sums of a column are added through
 // LDS: lane t sums 8 of column t/8's, three shuffle steps finish it.
 __global__ __launch_bounds__(256) void bwd_gemv_partial_kernel(
     const float* __restrict__ L, int64_t ldl, int64_t p0, int bI, int64_t n,
@@ -260,13 +261,13 @@ __global__ __launch_bounds__(256) void bwd_gemv_partial_kernel(
     const int c0 = blockIdx.x * (4 * BW_CPW) + wave * BW_CPW;
     const int j0 = blockIdx.z * NR;
     const int nj = nrhs - j0 < NR ? nrhs - j0 : NR;
-    float l[BW_CPW][8];
+    float l[BW_CPW][BW_RPL];
 #pragma unroll
     for (int cc = 0; cc < BW_CPW; ++cc) {
         const int c = c0 + cc < bI ? c0 + cc : bI - 1;
         const float* col = L + (p0 + c) * ldl;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < BW_RPL; ++k) {
             const int64_t r = rs + lane + 64 * k;
             l[cc][k] = col[r < n ? r : n - 1];
         }
@@ -275,9 +276,9 @@ __global__ __launch_bounds__(256) void bwd_gemv_partial_kernel(
     for (int j = 0; j < NR; ++j) {
         if (j < nj) {
             const double* x = X + (int64_t)(j0 + j) * ldx;
-            double xv[8];
+            double xv[BW_RPL];
 #pragma unroll
-            for (int k = 0; k < 8; ++k) {
+            for (int k = 0; k < BW_RPL; ++k) {
                 const int64_t r = rs + lane + 64 * k;
                 const double v = x[r < n ? r : n - 1];
                 xv[k] = r < n ? v : 0.0;                     // rows beyond n weigh nothing
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(256) void bwd_gemv_partial_kernel(
             for (int cc = 0; cc < BW_CPW; ++cc) {
                 double a = 0.0;
 #pragma unroll
-                for (int k = 0; k < 8; ++k) a = fma((double)l[cc][k], xv[k], a);
+                for (int k = 0; k < BW_RPL; ++k) a = fma((double)l[cc][k], xv[k], a);
                 buf[wave][cc][lane] = a;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
