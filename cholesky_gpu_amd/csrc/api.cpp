@@ -34,7 +34,7 @@ extern "C" const char* mpc_version_string(void) { return "mpchol 0.1 (gfx950)"; 
 int mpc_ws_reserve(mpc_workspace& w, size_t bytes) {
     if (bytes <= w.bytes && w.ptr) return 0;
     if (w.ptr) {
-        hipFree(w.ptr);
+        (void)hipFree(w.ptr);
         w.ptr = nullptr;
         w.bytes = 0;
     }
@@ -141,33 +141,33 @@ static int wait_winv(mpc_handle* h, hipStream_t s) {
 }
 
 static void ws_free(mpc_workspace& w) {
-    if (w.ptr) hipFree(w.ptr);
+    if (w.ptr) (void)hipFree(w.ptr);
     w.ptr = nullptr;
     w.bytes = 0;
 }
 
 extern "C" int mpc_handle_destroy(mpc_handle* h) {
     if (!h) return 0;
-    hipSetDevice(h->device);
-    if (h->stream) hipStreamSynchronize(h->stream);
-    if (h->pstream) hipStreamSynchronize(h->pstream);
-    if (h->wstream) hipStreamSynchronize(h->wstream);
-    if (h->fstream) hipStreamSynchronize(h->fstream);
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->pstream) (void)hipStreamSynchronize(h->pstream);
+    if (h->wstream) (void)hipStreamSynchronize(h->wstream);
+    if (h->fstream) (void)hipStreamSynchronize(h->fstream);
     for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
                              &h->pcg, &h->scratch, &h->winv, &h->gp64, &h->gp32, &h->gpvec})
         ws_free(*w);
-    if (h->pcg_exec) hipGraphExecDestroy(h->pcg_exec);
-    if (h->ev_col) hipEventDestroy(h->ev_col);
-    if (h->ev_panel) hipEventDestroy(h->ev_panel);
-    if (h->ev_wready) hipEventDestroy(h->ev_wready);
-    if (h->ev_wdone) hipEventDestroy(h->ev_wdone);
-    if (h->ev_wpre) hipEventDestroy(h->ev_wpre);
-    if (h->ev_bulk) hipEventDestroy(h->ev_bulk);
-    if (h->ev_far) hipEventDestroy(h->ev_far);
-    if (h->pstream) hipStreamDestroy(h->pstream);
-    if (h->wstream) hipStreamDestroy(h->wstream);
-    if (h->fstream) hipStreamDestroy(h->fstream);
-    if (h->own_stream) hipStreamDestroy(h->stream);
+    if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
+    if (h->ev_col) (void)hipEventDestroy(h->ev_col);
+    if (h->ev_panel) (void)hipEventDestroy(h->ev_panel);
+    if (h->ev_wready) (void)hipEventDestroy(h->ev_wready);
+    if (h->ev_wdone) (void)hipEventDestroy(h->ev_wdone);
+    if (h->ev_wpre) (void)hipEventDestroy(h->ev_wpre);
+    if (h->ev_bulk) (void)hipEventDestroy(h->ev_bulk);
+    if (h->ev_far) (void)hipEventDestroy(h->ev_far);
+    if (h->pstream) (void)hipStreamDestroy(h->pstream);
+    if (h->wstream) (void)hipStreamDestroy(h->wstream);
+    if (h->fstream) (void)hipStreamDestroy(h->fstream);
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return 0;
 }
@@ -175,7 +175,7 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
 extern "C" int mpc_set_stream(mpc_handle* h, void* s) {
     MPC_CHECK(h, "handle is NULL");
     MPC_HIP(hipStreamSynchronize(h->stream));
-    if (h->own_stream) hipStreamDestroy(h->stream);
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
     if (s) {
         h->stream = (hipStream_t)s;
         h->own_stream = false;
@@ -721,13 +721,16 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
             }
             MPC_HIP(hipEventRecord(ev[4], st));
             MPC_HIP(hipEventSynchronize(ev[4]));
-            float ms;
-            hipEventElapsedTime(&ms, ev[0], ev[1]); tp += ms;
-            hipEventElapsedTime(&ms, ev[1], ev[2]); tt += ms;
-            hipEventElapsedTime(&ms, ev[2], ev[3]); tq += ms;
-            hipEventElapsedTime(&ms, ev[3], ev[4]); ts += ms;
+            auto elapsed = [](hipEvent_t a, hipEvent_t b) {
+                float t = 0.f;
+                return hipEventElapsedTime(&t, a, b) == hipSuccess ? t : 0.f;
+            };
+            tp += elapsed(ev[0], ev[1]);
+            tt += elapsed(ev[1], ev[2]);
+            tq += elapsed(ev[2], ev[3]);
+            ts += elapsed(ev[3], ev[4]);
         }
-        for (auto& e : ev) hipEventDestroy(e);
+        for (auto& e : ev) (void)hipEventDestroy(e);
         if (W && (rc = mpc_build_block_inverses_range(st, dA, lda, n, Bo, dinv, W, WT, 0, nI)))
             return rc;
     }
@@ -740,9 +743,9 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     MPC_HIP(hipMemcpyAsync(&hs, sm, offsetof(mpc_small, tile_amax), hipMemcpyDeviceToHost, st));
     MPC_HIP(hipStreamSynchronize(st));
     float ms_total = 0;
-    hipEventElapsedTime(&ms_total, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&ms_total, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     fi->ms_total += ms_total;
     fi->ms_potrf += tp;
     fi->ms_trsm += tt;
@@ -1214,9 +1217,9 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         betas.assign(hab.begin() + hmax, hab.begin() + hmax + mh);
     }
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     mpc_pcg_info pi;
     std::memset(&pi, 0, sizeof pi);
     pi.iters = *std::max_element(iters.begin(), iters.end());
