@@ -53,6 +53,23 @@ int mpc_ws_reserve(mpc_workspace& w, size_t bytes) {
 
 // ---------------------------------------------------------------- small init kernels
 namespace {
+// N events owned by one call: destroyed on every return path
+template <int N>
+struct owned_events {
+    hipEvent_t e[N] = {};
+    owned_events() = default;
+    owned_events(const owned_events&) = delete;
+    owned_events& operator=(const owned_events&) = delete;
+    ~owned_events() {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    int create() {
+        for (auto& x : e) MPC_HIP(hipEventCreate(&x));
+        return 0;
+    }
+};
+
 __global__ void init_small_kernel(mpc_small* sm) {
     if (threadIdx.x == 0) {
         sm->info = 0;
@@ -595,9 +612,9 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     };
 
     double tp = 0, tt = 0, tq = 0, ts = 0;
-    hipEvent_t e0, e1;
-    MPC_HIP(hipEventCreate(&e0));
-    MPC_HIP(hipEventCreate(&e1));
+    owned_events<2> tev;
+    if ((rc = tev.create())) return rc;
+    hipEvent_t e0 = tev.e[0], e1 = tev.e[1];
     MPC_HIP(hipEventRecord(e0, st));
     if (!prof && AG > 1) {
         // One-deep look-ahead with aggregated far updates.  Block columns
@@ -689,8 +706,9 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
     } else {
         // sequential with per-phase events
-        hipEvent_t ev[5] = {};
-        for (auto& e : ev) MPC_HIP(hipEventCreate(&e));
+        owned_events<5> pev;
+        if ((rc = pev.create())) return rc;
+        hipEvent_t* ev = pev.e;
         for (int64_t j = 0; j < nt; ++j) {
             const int64_t c0 = j * nb, bs = std::min<int64_t>(nb, n - c0), e = c0 + bs, s = n - e;
             const int ntl = (int)((s + tr - 1) / tr), b = (int)(j & 1);
@@ -730,7 +748,6 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
             tq += elapsed(ev[2], ev[3]);
             ts += elapsed(ev[3], ev[4]);
         }
-        for (auto& e : ev) (void)hipEventDestroy(e);
         if (W && (rc = mpc_build_block_inverses_range(st, dA, lda, n, Bo, dinv, W, WT, 0, nI)))
             return rc;
     }
@@ -744,8 +761,6 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     MPC_HIP(hipStreamSynchronize(st));
     float ms_total = 0;
     (void)hipEventElapsedTime(&ms_total, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     fi->ms_total += ms_total;
     fi->ms_potrf += tp;
     fi->ms_trsm += tt;
@@ -1020,9 +1035,9 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     int* d_cand = b.active + nrhs;
     int* d_iters = d_cand + nrhs;
 
-    hipEvent_t e0, e1;
-    MPC_HIP(hipEventCreate(&e0));
-    MPC_HIP(hipEventCreate(&e1));
+    owned_events<2> tev;
+    if ((rc = tev.create())) return rc;
+    hipEvent_t e0 = tev.e[0], e1 = tev.e[1];
     MPC_HIP(hipEventRecord(e0, st));
     hipLaunchKernelGGL(fill_int_kernel, dim3((nrhs + 255) / 256), dim3(256), 0, st, b.active, nrhs, 1);
     // B into a packed copy for the norms
@@ -1218,8 +1233,6 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
     }
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     mpc_pcg_info pi;
     std::memset(&pi, 0, sizeof pi);
     pi.iters = *std::max_element(iters.begin(), iters.end());
