@@ -67,6 +67,12 @@ class GpInfo(C.Structure):
                 ("factor", FactorInfo), ("pcg", PcgInfo)]
 
 
+class Settings(C.Structure):
+    _fields_ = [("n", C.c_int64), ("condition_number", C.c_double), ("dist_type", C.c_int),
+                ("block_size", C.c_int), ("eps_prime", C.c_double), ("floor", C.c_double),
+                ("diag_pert", C.c_int), ("left", C.c_int)]
+
+
 P = C.c_void_p
 I64 = C.c_int64
 D = C.c_double
@@ -105,6 +111,14 @@ SIGNATURES = {
     "mpc_apply_precond": (I, [P, P, I64, I64, I, P, I64, P, I64]),
     "mpc_potrf_block": (I, [P, P, I, I64, C.POINTER(I)]),
     "mpc_convert_f8": (I, [P, P, P, I64, I]),
+    "mpc_pcg_last_coeffs": (I, [P, C.POINTER(D), C.POINTER(D), I, C.POINTER(I)]),
+    "mpc_settings_default": (None, [C.POINTER(Settings)]),
+    "mpc_settings_parse": (I, [C.c_char_p, C.c_size_t, C.POINTER(Settings)]),
+    "mpc_settings_read": (I, [C.c_char_p, C.POINTER(Settings)]),
+    "mpc_settings_apply": (I, [C.POINTER(Settings), C.POINTER(MatgenOpts), C.POINTER(FactorOpts)]),
+    "mpc_idx_header": (I, [C.c_char_p, C.POINTER(I), C.POINTER(I), C.POINTER(I64), C.POINTER(I64)]),
+    "mpc_idx_read_u8": (I, [C.c_char_p, I64, P, I64, C.POINTER(I64)]),
+    "mpc_idx_points": (I, [P, C.c_char_p, I64, P, I64, C.POINTER(I64), C.POINTER(I)]),
 }
 
 _lib = None

@@ -1231,6 +1231,8 @@ static int pcg_impl(mpc_handle* h, const pcg_operator& op, int64_t lda, const fl
         alphas.assign(hab.begin(), hab.begin() + mh);
         betas.assign(hab.begin() + hmax, hab.begin() + hmax + mh);
     }
+    h->last_alpha = alphas;
+    h->last_beta = betas;
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
     mpc_pcg_info pi;
@@ -1597,6 +1599,19 @@ extern "C" int mpc_potrf_block(mpc_handle* h, float* dA, int n, int64_t lda, int
     h->winv_owner = nullptr;
     if (info) *info = hi;
     return hi;
+}
+
+extern "C" int mpc_pcg_last_coeffs(mpc_handle* h, double* alphas, double* betas, int max,
+                                   int* count) {
+    MPC_CHECK(h && count && max >= 0, "pcg_last_coeffs: bad argument");
+    const int m = std::min<int>(max, (int)h->last_alpha.size());
+    MPC_CHECK(m == 0 || (alphas && betas), "pcg_last_coeffs: NULL output arrays");
+    for (int i = 0; i < m; ++i) {
+        alphas[i] = h->last_alpha[i];
+        betas[i] = h->last_beta[i];
+    }
+    *count = m;
+    return 0;
 }
 
 extern "C" int mpc_convert_f8(mpc_handle* h, const float* dx, uint8_t* dy, int64_t count, int prec) {

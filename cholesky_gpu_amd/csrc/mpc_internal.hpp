@@ -62,6 +62,8 @@ struct mpc_handle {
     // the last PCG iteration graph and the arguments it was captured with
     hipGraphExec_t        pcg_exec = nullptr;
     std::vector<uint64_t> pcg_key;
+    // CG coefficients of the last PCG call (first right-hand side)
+    std::vector<double>   last_alpha, last_beta;
     int          device = 0;
     hipStream_t  stream = nullptr;
     bool         own_stream = false;

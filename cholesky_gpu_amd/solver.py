@@ -341,3 +341,71 @@ class GPLoss(torch.autograd.Function):
 def gp_loss(ls, variance, noise, b, X, handle, opts=None):
     """Differentiable GP loss (see GPLoss); ls is a 1-element or d-element tensor."""
     return GPLoss.apply(ls, variance, noise, b, X, handle, opts)
+
+
+# ---------------------------------------------------------------- options file and idx data
+def settings_parse(text):
+    """Parse the JSON text of a settings file (the reference's settings.json
+    keys) into an L.Settings structure."""
+    lib = L.load()
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    s = L.Settings()
+    check(lib.mpc_settings_parse(raw, len(raw), C.byref(s)), "mpc_settings_parse")
+    return s
+
+
+def settings_read(path):
+    s = L.Settings()
+    check(L.load().mpc_settings_read(str(path).encode(), C.byref(s)), "mpc_settings_read")
+    return s
+
+
+def settings_options(s, mo=None, fo=None):
+    """(matgen_opts, factor_opts) configured from a settings structure:
+    defaults where the file says nothing, the file's values applied on top."""
+    lib = L.load()
+    if mo is None:
+        mo = L.MatgenOpts()
+        lib.mpc_matgen_opts_default(C.byref(mo))
+    if fo is None:
+        fo = factor_opts()
+    check(lib.mpc_settings_apply(C.byref(s), C.byref(mo), C.byref(fo)), "mpc_settings_apply")
+    return mo, fo
+
+
+def idx_header(path):
+    """(type_code, dims tuple, data_offset) of a big-endian idx file."""
+    t, nd = C.c_int(), C.c_int()
+    dims = (C.c_int64 * 4)()
+    off = C.c_int64()
+    check(L.load().mpc_idx_header(str(path).encode(), C.byref(t), C.byref(nd), dims, C.byref(off)),
+          "mpc_idx_header")
+    return t.value, tuple(dims[i] for i in range(nd.value)), off.value
+
+
+def idx_read_u8(path, count=-1):
+    """The first count items (all if count < 0) of an unsigned-byte idx file
+    as a numpy uint8 array of shape (items, *item_dims)."""
+    import numpy as np
+    _, dims, _ = idx_header(path)
+    m = dims[0] if count < 0 else min(count, dims[0])
+    per = int(np.prod(dims[1:])) if len(dims) > 1 else 1
+    out = np.empty(m * per, np.uint8)
+    items = C.c_int64()
+    check(L.load().mpc_idx_read_u8(str(path).encode(), m, out.ctypes.data_as(C.c_void_p),
+                                   out.size, C.byref(items)), "mpc_idx_read_u8")
+    return out.reshape((items.value,) + tuple(dims[1:]))
+
+
+def idx_points(h, path, n=-1):
+    """Images of an idx3 file as an (n, rows*cols) FP32 device tensor of
+    pixel / 255 (the real-data form of the image-shaped RBF points)."""
+    _, dims, _ = idx_header(path)
+    m = dims[0] if n < 0 else min(n, dims[0])
+    d = dims[1] * dims[2]
+    X = torch.empty(m, d, dtype=torch.float32, device=h.device)
+    n_out, d_out = C.c_int64(), C.c_int()
+    _pre(h)
+    check(h.lib.mpc_idx_points(h.h, str(path).encode(), m, _ptr(X), d, C.byref(n_out),
+                               C.byref(d_out)), "mpc_idx_points")
+    return X

@@ -275,6 +275,54 @@ int mpc_potrf_block(mpc_handle* h, float* dA, int n, int64_t lda, int* info);
 /* element-wise FP32 -> 8-bit conversion of a device array (prec E4M3 or E5M2),
    used for the exhaustive conversion sweep */
 int mpc_convert_f8(mpc_handle* h, const float* dx, uint8_t* dy, int64_t count, int prec);
+/* The CG coefficients of the last PCG call on this handle (first right-hand
+   side): alphas[i], betas[i] for i < *count (count <= max).  kappa_est of
+   mpc_pcg_info is the ratio of the extreme eigenvalues of the Lanczos
+   tridiagonal they define (the condition estimate that replaces
+   estimateConditionNumber, src/matgen.cuh:9-95, for the preconditioned
+   operator).  Host arrays. */
+int mpc_pcg_last_coeffs(mpc_handle* h, double* alphas, double* betas, int max, int* count);
+
+/* ---- options file (settings.json, src/settings.json:1-14) -------------- */
+typedef struct {
+    int64_t n;                  /* matrix_settings.n */
+    double  condition_number;   /* matrix_settings.condition_number */
+    int     dist_type;          /* matrix_settings.dist_type as MPC_DIST_* */
+    int     block_size;         /* factorization_settings.block_size as written */
+    double  eps_prime;          /* factorization_settings.eps_prime */
+    double  floor;              /* factorization_settings.floor (the reference's
+                                   flr argument, unused by its solver) */
+    int     diag_pert;          /* factorization_settings.diag_pert (perturb_diag) */
+    int     left;               /* factorization_settings.left */
+} mpc_settings;
+void mpc_settings_default(mpc_settings* s);
+/* Parse the JSON text of a settings file (replaces set_matrix_params,
+   src/json_utils.hpp:2-32, which reads mismatched keys).  Values may be JSON
+   strings holding numbers (as in the reference's file) or numbers; keys are
+   accepted with underscores or spaces; absent keys keep the defaults. */
+int mpc_settings_parse(const char* text, size_t len, mpc_settings* out);
+int mpc_settings_read(const char* path, mpc_settings* out);
+/* Options from settings: matgen dist and cond; factor nb, agg_panels and
+   eps_prime.  block_size 64/128/256 -> nb; a * 256 (a = 2..4) -> nb 256 with
+   a panels per aggregated update (the reference's panel width is the K of its
+   update); anything else -> MPC_ERR_UNSUPPORTED.  Either output may be NULL. */
+int mpc_settings_apply(const mpc_settings* s, mpc_matgen_opts* mo, mpc_factor_opts* fo);
+
+/* ---- idx files (the MNIST data of src/MNIST_data) ---------------------- */
+/* Header of a big-endian idx file: element type code (0x08 unsigned byte,
+   ...), ndim (1..4), dims (unused entries 1), byte offset of the data.  The
+   file size must match the header exactly. */
+int mpc_idx_header(const char* path, int* type_code, int* ndim, int64_t dims[4],
+                   int64_t* data_offset);
+/* The first count items (all if count < 0) of an unsigned-byte idx file into
+   out (host, out_bytes long); *items receives the number read. */
+int mpc_idx_read_u8(const char* path, int64_t count, uint8_t* out, int64_t out_bytes,
+                    int64_t* items);
+/* The first n images (all if n < 0) of an idx3 image file as points: dX
+   (device, row-major, ld ldx >= rows*cols) receives pixel / 255 in FP32;
+   *n_out and *d_out the point count and dimension. */
+int mpc_idx_points(mpc_handle* h, const char* path, int64_t n, float* dX, int64_t ldx,
+                   int64_t* n_out, int* d_out);
 
 #ifdef __cplusplus
 }
