@@ -768,8 +768,10 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     fi->ms_syrk += ts;
     fi->min_pivot = (double)__builtin_bit_cast(float, hs.mu_bits);
     fi->info = hs.info;
-    if (hs.nan_flag && !hs.info) {
-        mpc_set_error("factor: NaN/Inf (or FP16 overflow) in a panel");
+    // a NaN or Inf takes precedence over the failing column it causes (a
+    // retry cannot remove it from A)
+    if (hs.nan_flag) {
+        mpc_set_error("factor: NaN/Inf in a panel or pivot (first failing column %d)", hs.info);
         return MPC_ERR_NAN;
     }
     return hs.info;

@@ -10,7 +10,8 @@
 //    kernels_macros.cuh:177-190) and folds the tile amax / max row sum of
 //    squares into per-tile slots with order-independent atomics.
 //  quantize: the panel becomes the SYRK operand: row-major k-contiguous
-//    payload plus one E8M0 byte per row per 32 columns (replaces
+//    payload (8-bit tiles in the matrix-core operand order of
+//    mpc_f8_offset) plus one E8M0 byte per row per 32 columns (replaces
 //    convert_fp32_to_mxe4m3 / convert_transpose_fp32_to_mxe4m3,
 //    fp8_utils.cuh:27-130, and convertFloatToHalf, kernels_macros.cuh:115-124).
 //    Each workgroup decides its tile's precision from the tile statistics;
@@ -190,11 +191,15 @@ struct rule_args {
     double eps_prime;
 };
 
-__device__ __forceinline__ int decide(const rule_args& ra, float maxrs, float mu) {
+// FP16 is stored unscaled: a tile with an entry above its largest finite
+// value (65504) takes the next format of the ladder
+__device__ __forceinline__ int decide(const rule_args& ra, float maxrs, float mu, float amax) {
     if (ra.update_prec >= 0) return ra.update_prec;
-    for (int q = 0; q < ra.n_ladder; ++q)
+    for (int q = 0; q < ra.n_ladder; ++q) {
+        if (ra.ladder[q] == MPC_PREC_FP16 && !(amax <= 65504.f)) continue;
         if (mpc_dev_unit_roundoff(ra.ladder[q]) * (double)maxrs <= ra.eps_prime * (double)mu)
             return ra.ladder[q];
+    }
     return ra.ladder[ra.n_ladder - 1];
 }
 
@@ -215,16 +220,28 @@ __global__ __launch_bounds__(256) void quantize_kernel(
         prec = prec_in[tile];
     } else {
         float mu = __uint_as_float(smc->mu_bits);
-        prec = decide(ra, __uint_as_float(tile_maxrs[tile]), mu);
+        prec = decide(ra, __uint_as_float(tile_maxrs[tile]), mu, __uint_as_float(tile_amax[tile]));
         if (r0 % tile_rows == 0 && threadIdx.x == 0) {
             if (prec_out) prec_out[tile] = (uint8_t)prec;
             if (map_col) map_col[tile] = (uint8_t)prec;
         }
     }
-    if (row >= m) return;
     const int kb = k >> 5;
-    const float tam = __uint_as_float(tile_amax[tile]);
     const int esz = mpc_dev_elem_bytes(prec);
+    if (row >= m) {
+        // rows of the last 32-row block beyond m: zero slots in the 8-bit
+        // layout, where a wave reads whole blocks
+        if (esz == 1) {
+            for (int qq = qq0; qq < kb; qq += 8) {
+                *reinterpret_cast<uint4*>(payload + mpc_f8_offset(row, 32 * qq, row_stride)) =
+                    make_uint4(0u, 0u, 0u, 0u);
+                *reinterpret_cast<uint4*>(payload + mpc_f8_offset(row, 32 * qq + 16, row_stride)) =
+                    make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+        return;
+    }
+    const float tam = __uint_as_float(tile_amax[tile]);
     uint8_t* prow = payload + row * row_stride;
     bool bad = false;
     for (int qq = qq0; qq < kb; qq += 8) {
@@ -265,8 +282,10 @@ __global__ __launch_bounds__(256) void quantize_kernel(
                 float a2 = ldexpf(v[c + 2], -e), a3 = ldexpf(v[c + 3], -e);
                 w[c >> 2] = mpc_dev_pack4_f8(a0, a1, a2, a3, prec);
             }
-            *reinterpret_cast<uint4*>(dst) = make_uint4(w[0], w[1], w[2], w[3]);
-            *reinterpret_cast<uint4*>(dst + 16) = make_uint4(w[4], w[5], w[6], w[7]);
+            *reinterpret_cast<uint4*>(payload + mpc_f8_offset(row, 32 * qq, row_stride)) =
+                make_uint4(w[0], w[1], w[2], w[3]);
+            *reinterpret_cast<uint4*>(payload + mpc_f8_offset(row, 32 * qq + 16, row_stride)) =
+                make_uint4(w[4], w[5], w[6], w[7]);
         }
     }
     if (bad) atomicOr(nan_flag, 1);

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
     __shared__ float Dinv[SUBSZ];
     __shared__ __align__(16) float colb[2][32];     // column k of L by row parity
     __shared__ __align__(16) float Lc[32 * 32];     // L's columns, rows split by parity
-    __shared__ int fail_col;
+    __shared__ int fail_col, fail_nan;
     __shared__ float red[256];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
         st_amax[t] = 0u;
         st_maxrs[t] = 0u;
     }
-    if (tid == 0) fail_col = -1;
+    if (tid == 0) { fail_col = -1; fail_nan = 0; }
     const int nsb = (bs + 31) >> 5;
 
     // ---- load lower sub-tiles (padding: identity beyond bs), with many loads
@@ -393,9 +393,12 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
             }
             {
                 const uint64_t badm = __builtin_amdgcn_ballot_w64(!(pv > 0.f)) & 0xffffffffull;
+                const uint64_t nanm = __builtin_amdgcn_ballot_w64(!(pv == pv)) & 0xffffffffull;
                 const int bad = badm ? __builtin_ctzll(badm) : -1;
-                if (bad >= 0 && lane == 0 && kb * 32 + bad < bs && fail_col < 0)
+                if (bad >= 0 && lane == 0 && kb * 32 + bad < bs && fail_col < 0) {
                     fail_col = kb * 32 + bad;
+                    fail_nan = (int)((nanm >> bad) & 1u);     // the failing pivot is NaN
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -524,6 +527,7 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
     if (tid == 0) {
         if (fail_col >= 0) {
             sm->info = col0 + fail_col + 1;
+            if (fail_nan) atomicOr(&sm->nan_flag, 1);
         } else {
             atomicMin(&sm->mu_bits, __float_as_uint(red[0]));
         }

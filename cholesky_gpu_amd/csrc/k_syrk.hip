@@ -97,7 +97,7 @@ __device__ __forceinline__ void load8_f32(const operand_src& o, int64_t row, int
             }
         }
     } else {
-        uint2 a = *reinterpret_cast<const uint2*>(p + k0);
+        uint2 a = *reinterpret_cast<const uint2*>(o.payload + mpc_f8_offset(row, k0, o.row_stride));
         float sc = mpc_dev_exp2i((int)o.scales[row * o.kb + (k0 >> 5)] - 127);
         v2f t0, t1, t2, t3;
         if (mpc_dev_is_e4m3(o.prec)) {
@@ -175,41 +175,48 @@ struct tile_ctx {
 // BLAS GEMM with beta = 1 does: seeding the accumulator with -C instead would
 // round every one of the k FMA steps at the scale of |C| rather than of the
 // partial sum, about sqrt(k) times the error for a trailing update.
+//
+// C's old values are loaded (load_c) before the K loop, so their HBM latency
+// overlaps the operand loads and the products; store_c subtracts and
+// stores.  C is streamed: its loads and stores carry the non-temporal cache
+// policy (CPOL), so the lines of C, touched once per update, do not evict
+// the panel rows that the other tiles of the XCD re-read from L2.
+constexpr int kCpolStream = 2;     // nt: non-temporal (streaming) policy bit
+
 template <int QA>
-__device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[QA][QA]) {
+struct c_frag {
+    float v[QA][QA][16];
+};
+
+// Buffer addressing: the wave's base in the resource (scalar), the column
+// offset (32b + 8(i/4) + i%4) * ldc * 4 as the scalar soffset and the lane's
+// (32a + n + 4h ldc) * 4 as the one 32-bit VGPR per a.  With global
+// addressing every load would keep its own 64-bit VGPR address alive until
+// the matching store.
+template <int QA>
+__device__ __forceinline__ void load_c(const tile_ctx& t, c_frag<QA>& c) {
     constexpr int QE = 32 * QA;
     const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
-    // Buffer addressing: the wave's base in the resource (scalar), the
-    // column offset (32b + 8(i/4) + i%4) * ldc * 4 as the scalar soffset and
-    // the lane's (32a + n + 4h ldc) * 4 as the one 32-bit VGPR per a.  With
-    // global addressing every load would keep its own 64-bit VGPR address
-    // alive until the matching store.
-    const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc(t.C + t.rowbase + t.colbase * t.ldc, 0, 0x7fffffff,
-                                          0x00020000);
     const __amdgpu_buffer_rsrc_t rsl = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float*>(t.Cs) + t.rowbase + t.colbase * t.ldc, 0, 0x7fffffff, 0x00020000);
     const int ld4 = (int)t.ldc * 4;
     const int rlim = (int)(t.m - t.rowbase), clim = (int)(t.m - t.colbase);
-    int loff[QA];
-#pragma unroll
-    for (int a = 0; a < QA; ++a) loff[a] = ((32 * a + n) + 4 * h * (int)t.ldc) * 4;
-    // all loads first (one HBM latency), then subtract and store.  Loads are
-    // unconditional (a per-lane conditional load becomes a branch with its
-    // own wait): interior quadrants read every element (the diagonal
-    // quadrant's upper part is a valid address, masked at the store); edge
-    // quadrants read from a clamped row and column.
-    float v[QA][QA][16];
+    // Loads are unconditional (a per-lane conditional load becomes a branch
+    // with its own wait): interior quadrants read every element (the
+    // diagonal quadrant's upper part is a valid address, masked at the
+    // store); edge quadrants read from a clamped row and column.
     const bool interior = rlim >= QE && clim >= QE;
     if (interior) {
 #pragma unroll
-        for (int a = 0; a < QA; ++a)
+        for (int a = 0; a < QA; ++a) {
+            const int loff = ((32 * a + n) + 4 * h * (int)t.ldc) * 4;
 #pragma unroll
             for (int b = 0; b < QA; ++b)
 #pragma unroll
                 for (int i = 0; i < 16; ++i)
-                    v[a][b][i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                        rsl, loff[a], (32 * b + 8 * (i >> 2) + (i & 3)) * ld4, 0));
+                    c.v[a][b][i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                        rsl, loff, (32 * b + 8 * (i >> 2) + (i & 3)) * ld4, kCpolStream));
+        }
     } else {
 #pragma unroll
         for (int a = 0; a < QA; ++a)
@@ -217,25 +224,38 @@ __device__ __forceinline__ void update_c(const tile_ctx& t, const v16f acc[QA][Q
             for (int b = 0; b < QA; ++b)
 #pragma unroll
                 for (int i = 0; i < 16; ++i) {
-                    const int r = 32 * a + n, c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
-                    const int rc = r < rlim ? r : rlim - 1, cc = c < clim ? c : clim - 1;
-                    v[a][b][i] = __uint_as_float(
-                        __builtin_amdgcn_raw_buffer_load_b32(rsl, rc * 4 + cc * ld4, 0, 0));
+                    const int r = 32 * a + n, cc0 = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
+                    const int rc = r < rlim ? r : rlim - 1, cc = cc0 < clim ? cc0 : clim - 1;
+                    c.v[a][b][i] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                        rsl, rc * 4 + cc * ld4, 0, kCpolStream));
                 }
     }
+}
+
+template <int QA>
+__device__ __forceinline__ void store_c(const tile_ctx& t, const c_frag<QA>& c,
+                                        const v16f acc[QA][QA]) {
+    const int lane = threadIdx.x & 63, h = lane >> 5, n = lane & 31;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc(t.C + t.rowbase + t.colbase * t.ldc, 0, 0x7fffffff,
+                                          0x00020000);
+    const int ld4 = (int)t.ldc * 4;
+    const int rlim = (int)(t.m - t.rowbase), clim = (int)(t.m - t.colbase);
 #pragma unroll
-    for (int a = 0; a < QA; ++a)
+    for (int a = 0; a < QA; ++a) {
+        const int loff = ((32 * a + n) + 4 * h * (int)t.ldc) * 4;
 #pragma unroll
         for (int b = 0; b < QA; ++b)
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const int r = 32 * a + n, c = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
-                const bool ok = r < rlim && c < clim && (!t.diag || r >= c);
+                const int r = 32 * a + n, cc = 32 * b + 8 * (i >> 2) + 4 * h + (i & 3);
+                const bool ok = r < rlim && cc < clim && (!t.diag || r >= cc);
                 if (ok)
                     __builtin_amdgcn_raw_buffer_store_b32(
-                        __float_as_uint(v[a][b][i] - acc[a][b][i]), rs, loff[a],
-                        (32 * b + 8 * (i >> 2) + (i & 3)) * ld4, 0);
+                        __float_as_uint(c.v[a][b][i] - acc[a][b][i]), rs, loff,
+                        (32 * b + 8 * (i >> 2) + (i & 3)) * ld4, kCpolStream);
             }
+    }
 }
 
 // operand rows of the wave: column side (ra) and row side (rb), clamped to m
@@ -264,6 +284,17 @@ __device__ __forceinline__ void k_loop(int k, F&& body) {
     }
 }
 
+// K chunks of the 8-bit products kept in flight ahead of the chunk being
+// multiplied (register buffers; the runtime-K loop is otherwise bound by one
+// operand-load latency per 64-wide chunk)
+constexpr int kFp8Ahead = 0;
+
+template <int QA>
+struct f8_frag {
+    v8i A[QA], B[QA];
+    int sa[QA], sb[QA];
+};
+
 // FP8 class: A operand = column-side rows, B operand = row-side rows.
 template <int QA, int FA, int FB, int KC>
 __device__ __forceinline__ void mm_fp8(const operand_src& oc, const operand_src& orow,
@@ -277,29 +308,63 @@ __device__ __forceinline__ void mm_fp8(const operand_src& oc, const operand_src&
     // h = 1.  So lane half h carries K [kc + 16h, +16) in bytes 0-15 and
     // K [kc + 32 + 16h, +16) in bytes 16-31; K block h then spans exactly the
     // 32 elements the hardware scales with lane half h's scale byte.
-    k_loop<KC, 64>(k, [&](int kc) {
-        v8i A[QA], B[QA];
-        int sa[QA], sb[QA];
+    // the 32-row operand blocks of the wave (the lane's own 32 bytes of a
+    // chunk: mpc_f8_offset's layout)
+    const int64_t lane32 = (int64_t)(threadIdx.x & 63) * 32;
+    const uint8_t* ba[QA];
+    const uint8_t* bb[QA];
+#pragma unroll
+    for (int q = 0; q < QA; ++q) {
+        ba[q] = oc.payload + (t.colbase + 32 * q) * oc.row_stride + lane32;
+        bb[q] = orow.payload + (t.rowbase + 32 * q) * orow.row_stride + lane32;
+    }
+    auto load = [&](int kc, f8_frag<QA>& f) {
         const int koff = kc + 32 * h;          // scale block of this lane half
-        const int kl = kc + 16 * h;            // first K of bytes 0-15
+        const int64_t co = (int64_t)(kc >> 6) * 2048;
 #pragma unroll
         for (int q = 0; q < QA; ++q) {
-            const uint8_t* pa = oc.payload + ra[q] * oc.row_stride + kl;
-            const uint8_t* pb = orow.payload + rb[q] * orow.row_stride + kl;
-            uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 32);
-            uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 32);
-            A[q] = v8i{(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
-            B[q] = v8i{(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
-            sa[q] = oc.scales[ra[q] * oc.kb + (koff >> 5)];
-            sb[q] = orow.scales[rb[q] * orow.kb + (koff >> 5)];
+            const uint8_t* pa = ba[q] + co;
+            const uint8_t* pb = bb[q] + co;
+            uint4 a0 = *reinterpret_cast<const uint4*>(pa), a1 = *reinterpret_cast<const uint4*>(pa + 16);
+            uint4 b0 = *reinterpret_cast<const uint4*>(pb), b1 = *reinterpret_cast<const uint4*>(pb + 16);
+            f.A[q] = v8i{(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+            f.B[q] = v8i{(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+            f.sa[q] = oc.scales[ra[q] * oc.kb + (koff >> 5)];
+            f.sb[q] = orow.scales[rb[q] * orow.kb + (koff >> 5)];
         }
+    };
+    auto mma = [&](const f8_frag<QA>& f) {
 #pragma unroll
         for (int a = 0; a < QA; ++a)
 #pragma unroll
             for (int b = 0; b < QA; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(
-                    A[b], B[a], acc[a][b], FA, FB, 0, sa[b], 0, sb[a]);
-    });
+                    f.A[b], f.B[a], acc[a][b], FA, FB, 0, f.sa[b], 0, f.sb[a]);
+    };
+    if constexpr (KC > 0) {
+        k_loop<KC, 64>(k, [&](int kc) {
+            f8_frag<QA> f;
+            load(kc, f);
+            mma(f);
+        });
+    } else {
+        // rolling buffer of kFp8Ahead + 1 chunks; past the end, the last
+        // chunk is loaded again (branch-free) and never multiplied
+        constexpr int NB = kFp8Ahead + 1;
+        f8_frag<QA> buf[NB];
+        const int klast = k - 64;
+#pragma unroll
+        for (int i = 0; i < kFp8Ahead; ++i) load(64 * i < klast ? 64 * i : klast, buf[i]);
+        for (int kc = 0; kc < k; kc += 64 * NB) {
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const int c = kc + 64 * i;                       // chunk multiplied now
+                const int cn = c + 64 * kFp8Ahead;               // chunk loaded now
+                load(cn < klast ? cn : klast, buf[(i + kFp8Ahead) % NB]);
+                if (c < k) mma(buf[i]);
+            }
+        }
+    }
 }
 
 template <int QA, bool BF, int KC>
@@ -550,30 +615,48 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(float* __restrict__ C, int
         }
         if (first) t.Cs = P.csrc;
     }
-    for (int q = 0; q < P.np; ++q) {
+    // the panels this launch applies to the tile (bit q: panel q)
+    auto operands = [&](int q, operand_src& orow, operand_src& ocol) {
         const int slot = (P.p0 + q) % P.slots;
         const int64_t roff = P.rows0 - (int64_t)q * P.nb;   // C row 0 within this panel
         const uint8_t* prec = P.prec + slot * 1024 + roff / tile_rows;
-        operand_src orow, ocol;
         orow.prec = prec[trow];
         ocol.prec = prec[tcol];
         orow.payload = ocol.payload = P.payload + slot * P.pstride + roff * row_stride;
         orow.scales = ocol.scales = P.scales + slot * P.sstride + roff * (k >> 5);
         orow.row_stride = ocol.row_stride = row_stride;
         orow.kb = ocol.kb = k >> 5;
-        const int cls = pair_class(orow.prec, ocol.prec);
-        const int fa = mpc_dev_is_e4m3(ocol.prec) ? 0 : 1, fb = mpc_dev_is_e4m3(orow.prec) ? 0 : 1;
-        const bool f32pair = orow.prec == MPC_PREC_FP32 && ocol.prec == MPC_PREC_FP32;
+    };
+    unsigned use = 0;
+    for (int q = 0; q < P.np; ++q) {
+        operand_src orow, ocol;
+        operands(q, orow, ocol);
         if (CLS >= 0) {            // this launch computes one class (and format pair) only
             // FP32 class: FA 1 both FP32, 2 FP32 against a bf16-exact
             // format (split products), 0 FP32 against FP16
+            const int cls = pair_class(orow.prec, ocol.prec);
+            const int fa = mpc_dev_is_e4m3(ocol.prec) ? 0 : 1, fb = mpc_dev_is_e4m3(orow.prec) ? 0 : 1;
+            const bool f32pair = orow.prec == MPC_PREC_FP32 && ocol.prec == MPC_PREC_FP32;
             const bool f16 = orow.prec == MPC_PREC_FP16 || ocol.prec == MPC_PREC_FP16;
             const int f32kind = f32pair ? 1 : (f16 ? 0 : 2);
             if (cls != CLS || (CLS == MPC_PREC_E4M3 && (fa != FA || fb != FB)) ||
                 (CLS == MPC_PREC_FP32 && f32kind != FA))
                 continue;
         }
-        any = true;
+        use |= 1u << q;
+    }
+    if (!use) return;              // no panel of this launch's class on this tile
+    // C's old values in flight during the products (64-row tiles only: at
+    // 128-row tiles the K loop's operand buffers need those registers)
+    constexpr bool kPrefetchC = QA == 1;
+    c_frag<QA> cv;
+    if constexpr (kPrefetchC) load_c<QA>(t, cv);
+    for (int q = 0; q < P.np; ++q) {
+        if (!((use >> q) & 1u)) continue;
+        operand_src orow, ocol;
+        operands(q, orow, ocol);
+        const int cls = pair_class(orow.prec, ocol.prec);
+        const int fa = mpc_dev_is_e4m3(ocol.prec) ? 0 : 1, fb = mpc_dev_is_e4m3(orow.prec) ? 0 : 1;
         const int c = CLS >= 0 ? CLS : cls;
         if (c == MPC_PREC_E4M3) {
             if (CLS >= 0) mm_fp8<QA, FA, FB, KC>(ocol, orow, t, k, acc);
@@ -593,8 +676,8 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(float* __restrict__ C, int
             mm_f32<QA, KC, false, 2>(ocol, orow, t, k, acc);
         }
     }
-    if (!any) return;              // no panel of this launch's class on this tile
-    update_c<QA>(t, acc);
+    if constexpr (!kPrefetchC) load_c<QA>(t, cv);
+    store_c<QA>(t, cv, acc);
 }
 
 }  // namespace

@@ -73,5 +73,19 @@ __device__ __forceinline__ uint32_t mpc_dev_pack4_f8(float a0, float a1, float a
     return (uint32_t)w;
 }
 
+// Payload layout of an 8-bit tile: per 32-row block and 64-wide K chunk,
+// 2 KB in the order the 32x32x64 matrix-core op takes one operand, so that a
+// wave reads a block's chunk with two 16-byte loads per lane that together
+// cover 2 KB of consecutive bytes.  Lane l = n + 32 h holds 32 bytes: bytes
+// 0-15 = row n, K [64c + 16h, +16); bytes 16-31 = row n, K [64c + 32 + 16h,
+// +16).  Block b's chunks start at its first row's offset (32 b) * row_stride
+// (row_stride >= 64 * ceil(k / 64) bytes); FP32 and 16-bit tiles stay
+// row-major.
+__device__ __forceinline__ int64_t mpc_f8_offset(int64_t row, int kk, int64_t row_stride) {
+    const int c = kk >> 6, kp = kk & 63;
+    const int lane = (int)(row & 31) + 32 * ((kp >> 4) & 1);
+    return (row & ~(int64_t)31) * row_stride + c * 2048 + lane * 32 + (kp >> 5) * 16 + (kp & 15);
+}
+
 // 2^e as a float for e in [-127, 127] (exact)
 __device__ __forceinline__ float mpc_dev_exp2i(int e) { return ldexpf(1.0f, e); }

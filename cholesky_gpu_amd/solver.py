@@ -174,6 +174,19 @@ def pcg(h, A, Lf, B, nb=256, opts=None, X=None):
     return X, pi, rc
 
 
+def pcg_last_coeffs(h, maxn=4096):
+    """(alphas, betas) of the first right-hand side of the handle's last PCG
+    solve, as numpy float64 arrays (the CG coefficients behind kappa_est)."""
+    import numpy as np
+    a = np.zeros(maxn)
+    b = np.zeros(maxn)
+    cnt = C.c_int(0)
+    check(h.lib.mpc_pcg_last_coeffs(h.h, a.ctypes.data_as(C.POINTER(C.c_double)),
+                                    b.ctypes.data_as(C.POINTER(C.c_double)), maxn, C.byref(cnt)),
+          "mpc_pcg_last_coeffs")
+    return a[:cnt.value].copy(), b[:cnt.value].copy()
+
+
 def precond_cg(h, A, B, fopts=None, popts=None, X=None):
     """Factor a workspace copy of A, then PCG.  Returns (X, FactorInfo, PcgInfo, rc)."""
     n = A.shape[0]
@@ -194,7 +207,9 @@ def precond_cg(h, A, B, fopts=None, popts=None, X=None):
 # ---------------------------------------------------------------- building blocks
 def quantize_panel(h, P, prec_of_tile, tile_rows):
     """P: (k, m) tensor holding the column-major m x k panel.  Returns
-    (payload (m, 4k) uint8, scales (m, k/32) uint8)."""
+    (payload (m + 128, 4k) uint8, scales (m, k/32) uint8, tile formats).  An
+    8-bit tile's payload is in the matrix-core operand order (whole 32-row
+    blocks, hence the spare rows); other tiles are row-major."""
     k, m = P.shape
     pt = torch.as_tensor(list(prec_of_tile), dtype=torch.uint8).to(h.device)
     pay = torch.zeros(m + 128, 4 * k, dtype=torch.uint8, device=h.device)
@@ -202,7 +217,7 @@ def quantize_panel(h, P, prec_of_tile, tile_rows):
     _pre(h)
     check(h.lib.mpc_quantize_panel(h.h, _ptr(_dev(P)), m, k, m, tile_rows, _ptr(pt), _ptr(pay),
                                    4 * k, _ptr(sc)), "mpc_quantize_panel")
-    return pay[:m], sc[:m], pt
+    return pay, sc[:m], pt
 
 
 def syrk(h, Cm, payload, scales, k, tile_rows, prec_of_tile):
@@ -341,6 +356,35 @@ class GPLoss(torch.autograd.Function):
 def gp_loss(ls, variance, noise, b, X, handle, opts=None):
     """Differentiable GP loss (see GPLoss); ls is a 1-element or d-element tensor."""
     return GPLoss.apply(ls, variance, noise, b, X, handle, opts)
+
+
+class SpdSolve(torch.autograd.Function):
+    """X = A^-1 B for a fixed SPD A (FP32, device) and right-hand sides B
+    ((nrhs, n) float64), differentiable with respect to B.  Forward: PCG
+    preconditioned by the mixed-precision factor Lf of A.  Backward: A is
+    symmetric, so dL/dB = A^-1 dL/dX, one more PCG with the same factor (the
+    adjoint solve; no new factorisation)."""
+
+    @staticmethod
+    def forward(ctx, B, A, Lf, handle, nb, opts):
+        X, pi, rc = pcg(handle, A, Lf, B, nb=nb, opts=opts)
+        if rc != 0:
+            raise RuntimeError(f"SpdSolve: PCG did not converge ({info_dict(pi)})")
+        ctx.A, ctx.Lf, ctx.handle, ctx.nb, ctx.opts = A, Lf, handle, nb, opts
+        ctx.info = pi
+        return X.reshape(B.shape)
+
+    @staticmethod
+    def backward(ctx, G):
+        Gb, pi, rc = pcg(ctx.handle, ctx.A, ctx.Lf, G.contiguous(), nb=ctx.nb, opts=ctx.opts)
+        if rc != 0:
+            raise RuntimeError(f"SpdSolve: adjoint PCG did not converge ({info_dict(pi)})")
+        return Gb.reshape(G.shape), None, None, None, None, None
+
+
+def spd_solve(B, A, Lf, handle, nb=256, opts=None):
+    """Differentiable solve A X = B with the factor Lf (see SpdSolve)."""
+    return SpdSolve.apply(B, A, Lf, handle, nb, opts or pcg_opts())
 
 
 # ---------------------------------------------------------------- options file and idx data

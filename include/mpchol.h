@@ -242,9 +242,15 @@ int mpc_gp_loss_grad(mpc_handle* h, const float* dX, int64_t n, int d, int64_t l
 
 /* ---- building blocks, exported for tests and advanced callers --------- */
 /* Quantise an m x k panel (column-major, ld) in row tiles of tile_rows rows;
-   prec_of_tile is a device array of ceil(m/tile_rows) codes.  Payload is
-   row-major k-contiguous with row_stride bytes per row; scales is m x (k/32)
-   E8M0 bytes.  k must be a multiple of 32. */
+   prec_of_tile is a device array of ceil(m/tile_rows) codes.  FP32 and 16-bit
+   tiles: payload row-major k-contiguous with row_stride bytes per row.  8-bit
+   tiles: the matrix-core operand order, per 32-row block b and 64-wide chunk
+   c a 2048-byte group at byte (32 b) * row_stride + 2048 c, in which lane
+   l = n + 32 h holds 32 bytes (bytes 0-15: row 32 b + n, k in [64 c + 16 h,
+   +16); bytes 16-31: k in [64 c + 32 + 16 h, +16)); rows of the last block
+   beyond m are zero, so the payload needs 32 * ceil(m / 32) rows and
+   row_stride >= 64 * ceil(k / 64).  scales is m x (k/32) E8M0 bytes.  k must
+   be a multiple of 32. */
 int mpc_quantize_panel(mpc_handle* h, const float* dP, int64_t m, int64_t k, int64_t ld,
                        int tile_rows, const uint8_t* d_prec_of_tile, uint8_t* d_payload,
                        int64_t row_stride, uint8_t* d_scales);

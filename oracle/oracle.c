@@ -24,6 +24,9 @@
  *   D3 Precision is chosen per panel tile by the min-pivot rule
  *      (orc_rule_minpivot) instead of can_switch's racy per-step atomicExch
  *      (kernels_macros.cuh:191-211, read on the host at switching_chol.cuh:124).
+ *      FP16, stored unscaled, is skipped for a tile whose largest magnitude
+ *      exceeds FP16's largest finite value 65504 (the next ladder format is
+ *      taken instead).
  *   D4 FP8 scales are powers of two, the smallest with amax/s <= max finite
  *      (no saturation); MX blocks are 32 consecutive elements along k of one
  *      row (fp8_utils.cuh:42-58 reduces the wrong lanes and scales to [-1,1]).
@@ -676,12 +679,19 @@ int orc_potrf_f32_micro(float* A, int n, int ld) {
     return 0;
 }
 
-int orc_rule_minpivot(const int* ladder, int n_ladder, double eps_prime,
-                      double max_row_sumsq, double min_pivot) {
-    for (int q = 0; q < n_ladder; ++q)
+int orc_rule_minpivot_amax(const int* ladder, int n_ladder, double eps_prime,
+                           double max_row_sumsq, double min_pivot, double tile_amax) {
+    for (int q = 0; q < n_ladder; ++q) {
+        if (ladder[q] == ORC_FP16 && !(tile_amax <= 65504.0)) continue;
         if (orc_unit_roundoff(ladder[q]) * max_row_sumsq <= eps_prime * min_pivot)
             return ladder[q];
+    }
     return ladder[n_ladder - 1];
+}
+
+int orc_rule_minpivot(const int* ladder, int n_ladder, double eps_prime,
+                      double max_row_sumsq, double min_pivot) {
+    return orc_rule_minpivot_amax(ladder, n_ladder, eps_prime, max_row_sumsq, min_pivot, 0.0);
 }
 
 /* compute class of a tile pair: FP32 if either is FP32, else the 16-bit class,
@@ -798,10 +808,15 @@ int orc_factor_mixed(float* A, int64_t n, int64_t lda, const orc_factor_opts* o,
             } else if (o->update_prec >= 0) {
                 p = o->update_prec;
             } else {
-                double mx = 0.0;
-                for (int64_t i = e + t * tr; i < e + (t + 1) * tr && i < n; ++i)
+                double mx = 0.0, am = 0.0;
+                for (int64_t i = e + t * tr; i < e + (t + 1) * tr && i < n; ++i) {
                     if (rs[i] > mx) mx = rs[i];
-                p = orc_rule_minpivot(o->ladder, o->n_ladder, o->eps_prime, mx, mu);
+                    for (int64_t c = 0; c < bs; ++c) {
+                        const double v = fabs((double)A[i + (c0 + c) * lda]);
+                        if (v > am) am = v;
+                    }
+                }
+                p = orc_rule_minpivot_amax(o->ladder, o->n_ladder, o->eps_prime, mx, mu, am);
             }
             ptile[t] = (uint8_t)p;
             if (map_out) map_out[(j + 1 + t) + j * nt] = (uint8_t)p;

@@ -145,7 +145,10 @@ typedef struct {
 int orc_factor_mixed(float* A, int64_t n, int64_t lda, const orc_factor_opts* o,
                      const uint8_t* map_in, uint8_t* map_out, float* updated_diag,
                      int max_steps, orc_factor_info* info);
-/* the switching rule for one tile */
+/* the switching rule for one tile (tile_amax: largest magnitude in the tile;
+   FP16 is skipped above 65504; orc_rule_minpivot: the rule without that test) */
+int orc_rule_minpivot_amax(const int* ladder, int n_ladder, double eps_prime,
+                           double max_row_sumsq, double min_pivot, double tile_amax);
 int orc_rule_minpivot(const int* ladder, int n_ladder, double eps_prime,
                       double max_row_sumsq, double min_pivot);
 

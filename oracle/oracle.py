@@ -107,6 +107,7 @@ def _declare(L):
         "orc_factor_mixed": (C.c_int, [P, I64, I64, C.POINTER(FactorOpts), P, P, P, C.c_int,
                                        C.POINTER(FactorInfo)]),
         "orc_rule_minpivot": (C.c_int, [P, C.c_int, D, D, D]),
+        "orc_rule_minpivot_amax": (C.c_int, [P, C.c_int, D, D, D, D]),
         "orc_pcg": (C.c_int, [P, I64, P, I64, I64, P, P, C.POINTER(PcgOpts), P, P, P]),
         "orc_lanczos_cond": (D, [P, P, C.c_int]),
         "orc_num_threads": (C.c_int, []),

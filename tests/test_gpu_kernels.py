@@ -84,6 +84,20 @@ def test_potrf_block_indefinite(handle):
 
 
 # ---------------------------------------------------------------- quantiser
+def f8_rowmajor(flat, r0, r1, k, row_stride):
+    """Rows r0..r1-1 (k codes each) of an 8-bit tile stored in the
+    matrix-core operand order: per 32-row block and 64-wide chunk c, lane
+    l = n + 32 h holds bytes 0-15 = row n, K [64c + 16h, +16) and bytes 16-31 =
+    row n, K [64c + 32 + 16h, +16), the block's chunks from its first row's
+    offset."""
+    rows = np.arange(r0, r1)[:, None]
+    kk = np.arange(k)[None, :]
+    c, kp = kk >> 6, kk & 63
+    lane = (rows & 31) + 32 * ((kp >> 4) & 1)
+    off = (rows & ~31) * row_stride + c * 2048 + lane * 32 + (kp >> 5) * 16 + (kp & 15)
+    return flat[off]
+
+
 ALL_FORMATS = [L.PREC_E4M3, L.PREC_E5M2, L.PREC_MXE4M3, L.PREC_MXE5M2, L.PREC_BF16,
                L.PREC_FP16, L.PREC_FP32]
 
@@ -102,10 +116,18 @@ def test_quantize_panel_bitexact(handle, oracle, m, k, tile_rows):
     pay_o, sc_o, _ = oracle.quantize_panel(P, precs, tile_rows)
     pay_g = pay_g.cpu().numpy()
     sc_g = sc_g.cpu().numpy()
+    flat = pay_g.reshape(-1)
     for t in range(nt):
         r0, r1 = t * tile_rows, min(m, (t + 1) * tile_rows)
         nbytes = k * {L.PREC_FP32: 4, L.PREC_BF16: 2, L.PREC_FP16: 2}.get(precs[t], 1)
-        assert np.array_equal(pay_g[r0:r1, :nbytes], pay_o[r0:r1, :nbytes]), L.PREC_NAMES[precs[t]]
+        got = pay_g[r0:r1, :nbytes]
+        if nbytes == k:                # 8-bit tile: matrix-core operand order
+            got = f8_rowmajor(flat, r0, r1, k, pay_g.shape[1])
+            # rows of the last 32-row block beyond m hold zeros
+            r32 = min(((r1 + 31) // 32) * 32, (t + 1) * tile_rows)
+            if r1 == m and r32 > m:
+                assert not f8_rowmajor(flat, m, r32, k, pay_g.shape[1]).any()
+        assert np.array_equal(got, pay_o[r0:r1, :nbytes]), L.PREC_NAMES[precs[t]]
         assert np.array_equal(sc_g[r0:r1], sc_o[r0:r1]), L.PREC_NAMES[precs[t]]
 
 
@@ -297,6 +319,39 @@ def test_factor_matches_oracle(handle, oracle, n, nb, prec, ladder, agg):
     for c in range(8):
         assert abs(fi.syrk_flops[c] - ofi.flops[c]) <= 1e-9 * sum(ofi.flops), (c, list(fi.syrk_flops),
                                                                               list(ofi.flops))
+
+
+@pytest.mark.parametrize("prec,agg", [
+    (L.PREC_E4M3, 2), (L.PREC_MXE4M3, 2), (L.PREC_E5M2, 1), (L.PREC_MXE5M2, 3),
+])
+def test_factor_fixed_fp8_matches_oracle(handle, oracle, prec, agg):
+    """Every trailing update in one 8-bit format (the all-FP8 factor of
+    fp8_chol.cuh:125-152, here with per-tile or per-32 E8M0 scales): the GPU
+    factor against the oracle's, tile formats all equal to the fixed one."""
+    n = 1536
+    say("factor fixed", L.PREC_NAMES[prec], agg)
+    A, _ = oracle.householder_spd(n, 1e2, 1e2, 23)
+    T = to_dev_cm(A)
+    opts = S.factor_opts(nb=256, update_prec=prec, ladder=(prec,), fallback=0, agg_panels=agg)
+    rc, fi, mp, dg = S.factor(handle, T, opts, want_map=True, want_diag=True)
+    assert rc == 0, rc
+    gmap = mp.cpu().numpy().T
+    low = np.tril(np.ones_like(gmap, bool), -1)
+    assert np.all(gmap[low] == prec)
+    assert fi.syrk_flops[L.PREC_E4M3] == sum(fi.syrk_flops)      # FP8 class only
+    Lg = np.tril(from_dev_cm(T))
+    Lo, rco, omap, udo, ofi = oracle.factor_mixed(A, nb=256, update_prec=prec, ladder=(prec,),
+                                                  tile_rows=256)
+    assert rco == 0
+    assert np.array_equal(omap[low], gmap[low])
+    Lo = np.tril(Lo)
+    rel = np.linalg.norm(Lg - Lo) / np.linalg.norm(Lo)
+    assert rel < 5e-3, rel
+    A64 = A.astype(np.float64)
+    bg = np.linalg.norm(Lg.astype(np.float64) @ Lg.T.astype(np.float64) - A64) / np.linalg.norm(A64)
+    bo = np.linalg.norm(Lo.astype(np.float64) @ Lo.T.astype(np.float64) - A64) / np.linalg.norm(A64)
+    assert bg < 2 * bo + 1e-6, (bg, bo)
+    assert abs(sum(fi.syrk_flops) - sum(ofi.flops)) <= 1e-9 * sum(ofi.flops)
 
 
 @pytest.mark.parametrize("n,nb,ladder,nrhs", [

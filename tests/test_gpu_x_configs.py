@@ -13,6 +13,7 @@ import pytest
 import torch
 
 from cholesky_gpu_amd import _lib as L
+from cholesky_gpu_amd import accuracy as ACC
 from cholesky_gpu_amd import solver as S
 
 pytestmark = pytest.mark.gpu
@@ -71,6 +72,12 @@ def test_c1_rbf_fp16_update(handle, oracle):
     assert bg < 2 * bo + 1e-6, (bg, bo)
     _, it_o, _, _ = oracle.pcg(Kh, Lgh, b.cpu().numpy()[0], tol=1e-8, maxit=200)
     assert abs(pi.iters - it_o) <= 2, (pi.iters, it_o)
+    rep = ACC.report(K, Lg, Xs, b, pi)
+    say("C1 accuracy", rep)
+    assert rep["bwd_err"] < 2 * bo + 1e-6
+    # ||x - x*|| / ||x*|| <= cond(K) * relres; cond(K) from the FP64 spectrum
+    ev = np.linalg.eigvalsh(K64)
+    assert rep["fwd_err"] <= ev.max() / ev.min() * rep["relres"] * 1.01, rep
 
 
 def test_c2_conditioned_e4m3(handle):
@@ -114,6 +121,17 @@ def test_c3_image_rbf_mxfp8(handle):
     assert rc == 0, (rc, S.info_dict(fi), S.info_dict(pi))
     rr = relres_fp64(K, Xs, B)
     assert np.all(rr <= 1e-10), rr
+    # accuracy report: the same factor in a buffer of its own (the solve's
+    # lives in the library's workspace), forward error against the device's
+    # FP64 Cholesky solve (a host one at n = 60000 would take minutes)
+    Lf = K.clone()
+    rc2, fi2, _, _ = S.factor(handle, Lf, fo)
+    assert rc2 == 0 and list(fi2.syrk_flops) == list(fi.syrk_flops)
+    rep = ACC.report(K, Lf, Xs, B, pi)
+    say("C3 accuracy", rep)
+    assert rep["bwd_err"] < 1e-2, rep
+    assert rep["fwd_err"] < 1e-3, rep
+    assert 1.0 <= rep["kappa_est"] < 1e4
 
 
 def test_c4_gp_full_size(handle, oracle):
@@ -159,3 +177,56 @@ def test_c5_switching_at_scale(handle):
     assert rc == 0, (rc, S.info_dict(fi), S.info_dict(pi))
     rr = relres_fp64(A, Xs, B)
     assert np.all(rr <= 1e-10), rr
+
+
+def test_c5_gradient_wrt_b(handle):
+    """C5's gradients with respect to b: loss = 1/2 sum_j b_j^T A^-1 b_j over
+    the 16 right-hand sides at n = 65536, differentiated through the
+    hand-written backward of solver.spd_solve (an adjoint PCG with the same
+    mixed-precision factor).  The exact gradient is A^-1 B = X: the autograd
+    gradient equals it to the solve's accuracy, and a central difference of
+    the loss along a random direction D matches <D, grad>."""
+    n, nrhs = 65536, 16
+    say("C5 grad b", n, nrhs)
+    A, lam = S.matgen_householder(handle, n, cond=1e4, lam_max=1.0, seed=65536, n_cluster=64,
+                                  cluster_lo=1e-8, cluster_hi=1.1e-8)
+    fo = S.factor_opts(nb=256, update_prec=L.PREC_SWITCH,
+                       ladder=(L.PREC_E5M2, L.PREC_MXE4M3, L.PREC_BF16, L.PREC_FP32),
+                       eps_prime=0.125)
+    Lf = A.clone()
+    rc, fi, _, _ = S.factor(handle, Lf, fo)
+    assert rc == 0
+    po = S.pcg_opts(tol=1e-10, maxit=1000)
+    B = torch.from_numpy(np.random.default_rng(5).standard_normal((nrhs, n))).cuda()
+    Bv = B.clone().requires_grad_(True)
+    X = S.spd_solve(Bv, A, Lf, handle, opts=po)
+    loss = 0.5 * (Bv * X).sum()
+    loss.backward()
+    G = Bv.grad.detach()
+    Xd = X.detach()
+    rel = float((G - Xd).norm() / Xd.norm())
+    say("C5 grad b: ||grad - X|| / ||X||", rel)
+    # both are A^-1 B to the PCG tolerance: residuals of 1e-10 each, which
+    # the conditioning (1e8) can turn into up to 1e-2 in the solution
+    assert rel < 2e-2, rel
+    assert np.all(relres_fp64(A, G, B) <= 1e-10)
+    # the loss is exactly quadratic in B, so a central difference has no
+    # truncation error at any step; a unit step keeps the solves' own
+    # errors (|x^T r| <= |x| |b| tol per column) far below <D, grad>
+    D = torch.from_numpy(np.random.default_rng(6).standard_normal((nrhs, n))).cuda()
+    eps = 1.0
+
+    def f(Bp):
+        with torch.no_grad():
+            return float(0.5 * (Bp * S.spd_solve(Bp, A, Lf, handle, opts=po)).sum())
+    fd = (f(B + eps * D) - f(B - eps * D)) / (2 * eps)
+    dd = float((D * G).sum())
+    say("C5 grad b: central difference", fd, "<D, grad>", dd)
+    assert abs(fd - dd) <= 1e-5 * abs(dd), (fd, dd)
+    del Bv, X, G
+    rep = ACC.report(A, Lf, Xd, B)
+    say("C5 accuracy", rep)
+    lamh = lam.cpu().numpy()
+    # ||x - x*|| / ||x*|| <= cond(A) * relres (cond from the generated spectrum)
+    assert rep["fwd_err"] <= lamh.max() / lamh.min() * rep["relres"] * 1.01, rep
+    assert rep["bwd_err"] < 1e-2, rep

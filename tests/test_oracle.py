@@ -350,3 +350,26 @@ def test_probes_are_rademacher_and_seeded():
     for p, i in [(0, 0), (3, 17), (7, 4095)]:
         u = O.uniform(99, 3, p * 4096 + i)
         assert Z[p, i] == (-1.0 if u < 0.5 else 1.0)
+
+
+def test_rule_skips_fp16_out_of_range():
+    """FP16 (stored unscaled) is taken only for a tile whose largest entry is
+    finite in FP16 (<= 65504); otherwise the next format of the ladder.  With
+    eps' = 1e6 the pivot test admits FP16 for every tile: unscaled, every
+    tile is FP16; scaled by 1e10, the tiles with entries beyond FP16's range
+    take FP32 and the factor completes without an overflow (the quantiser
+    would report a non-finite FP16 value)."""
+    lad = np.array([O.FP16, O.FP32], np.int32)
+    f = O.lib().orc_rule_minpivot_amax
+    assert f(lad.ctypes.data, 2, 1e6, 1.0, 1.0, 65504.0) == O.FP16
+    assert f(lad.ctypes.data, 2, 1e6, 1.0, 1.0, 65505.0) == O.FP32
+    assert f(lad.ctypes.data, 2, 1e6, 1.0, 1.0, float("nan")) == O.FP32
+    A, _ = O.householder_spd(512, 1e3, 1e3, 21)
+    low = np.tril(np.ones((4, 4), bool), -1)
+    for scale, all16 in ((1e10, False), (1.0, True)):
+        L, rc, mp, _, fi = O.factor_mixed(A * np.float32(scale), nb=128, update_prec=-1,
+                                          ladder=(O.FP16, O.FP32), eps_prime=1e6)
+        assert rc == 0
+        assert np.all(np.isfinite(np.tril(L)))
+        assert np.all(mp[low] == O.FP16) == all16
+        assert np.all((mp[low] == O.FP16) | (mp[low] == O.FP32))
