@@ -23,6 +23,7 @@ PREC_FP32, PREC_BF16, PREC_FP16, PREC_E5M2, PREC_E4M3, PREC_MXE4M3, PREC_MXE5M2 
 PREC_SWITCH = -1
 PREC_NAMES = {0: "fp32", 1: "bf16", 2: "fp16", 3: "e5m2", 4: "e4m3", 5: "mxe4m3", 6: "mxe5m2"}
 
+NOALLOC_MAX_TILES = 1024        # MPC_NOALLOC_MAX_TILES of include/mpchol.h
 DIST_RANDOM_LOG_UNIFORM, DIST_CLUSTERED, DIST_ARITHMETIC, DIST_GEOMETRIC, DIST_LOGSPACED = range(5)
 MAT_HOUSEHOLDER_SPD = 0
 
@@ -106,6 +107,7 @@ SIGNATURES = {
     "mpc_gp_loss_grad": (I, [P, P, I64, I, I64, C.POINTER(D), I, D, D, P, C.POINTER(GpOpts), P,
                              C.POINTER(D), C.POINTER(D), C.POINTER(D), C.POINTER(GpInfo)]),
     "mpc_quantize_panel": (I, [P, P, I64, I64, I64, I, P, P, I64, P]),
+    "mpc_quantize_panel_noalloc": (I, [P, P, I64, I64, I64, I, P, P, I64, P]),
     "mpc_syrk": (I, [P, P, I64, I64, I64, P, I64, P, I, P]),
     "mpc_syrk_masked": (I, [P, P, I64, I64, I64, P, I64, P, I, P, C.c_uint, C.c_uint]),
     "mpc_apply_precond": (I, [P, P, I64, I64, I, P, I64, P, I64]),

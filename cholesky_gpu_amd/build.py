@@ -26,7 +26,7 @@ FLAGS = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 SOURCES = ["api.cpp", "k_potrf.hip", "k_panel.hip", "k_syrk.hip", "k_pcg.hip", "k_matgen.hip",
-           "k_tri.hip", "k_gp.hip", "io.cpp"]
+           "k_tri.hip", "k_gp.hip", "io.cpp", "noalloc.hip"]
 
 
 def build(verbose=False):

@@ -143,6 +143,7 @@ extern "C" int mpc_handle_create(mpc_handle** out, int device) {
         rc = MPC_ERR_HIP;
     }
     if (!rc) rc = mpc_ws_reserve(h->small, sizeof(mpc_small));
+    if (!rc) rc = mpc_noalloc_init_device(device);
     if (rc) { mpc_handle_destroy(h); return rc; }
     *out = h;
     return 0;

@@ -123,6 +123,10 @@ __device__ __forceinline__ void mpc_lds_barrier() {
 // grow-only device allocation in a handle workspace
 int mpc_ws_reserve(mpc_workspace& w, size_t bytes);
 
+// noalloc.hip: looks up the current device's static scratch array (once per
+// device; mpc_handle_create calls it)
+int mpc_noalloc_init_device(int device);
+
 // ---------------------------------------------------------------- small device state
 // layout of handle->small (all device):
 struct mpc_small {

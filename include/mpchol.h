@@ -254,6 +254,17 @@ int mpc_gp_loss_grad(mpc_handle* h, const float* dX, int64_t n, int d, int64_t l
 int mpc_quantize_panel(mpc_handle* h, const float* dP, int64_t m, int64_t k, int64_t ld,
                        int tile_rows, const uint8_t* d_prec_of_tile, uint8_t* d_payload,
                        int64_t row_stride, uint8_t* d_scales);
+/* mpc_quantize_panel without any device allocation (no counterpart in the
+   reference): for callers that cannot allocate (code under stream capture,
+   serving loops, harnesses that forbid hipMalloc).  Same arguments, outputs
+   and errors as mpc_quantize_panel; its scratch (one word per row tile and a
+   flag) is a static per-device array of MPC_NOALLOC_MAX_TILES tiles, so
+   ceil(m / tile_rows) above that is rejected (MPC_ERR_ARG).  Calls that
+   overlap on one device race on that array. */
+#define MPC_NOALLOC_MAX_TILES 1024
+int mpc_quantize_panel_noalloc(mpc_handle* h, const float* dP, int64_t m, int64_t k, int64_t ld,
+                               int tile_rows, const uint8_t* d_prec_of_tile, uint8_t* d_payload,
+                               int64_t row_stride, uint8_t* d_scales);
 /* C (m x m lower, FP32, ldc) -= Q Q^T for a quantised panel from
    mpc_quantize_panel (k multiple of 64, tile_rows multiple of 128). */
 int mpc_syrk(mpc_handle* h, float* dC, int64_t m, int64_t ldc, int64_t k,
