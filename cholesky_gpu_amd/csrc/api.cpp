@@ -172,7 +172,8 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     if (h->wstream) (void)hipStreamSynchronize(h->wstream);
     if (h->fstream) (void)hipStreamSynchronize(h->fstream);
     for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
-                             &h->pcg, &h->scratch, &h->winv, &h->gp64, &h->gp32, &h->gpvec})
+                             &h->pcg, &h->scratch, &h->winv, &h->gp64, &h->gp32, &h->gpvec,
+                             &h->tlist})
         ws_free(*w);
     if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
     if (h->ev_col) (void)hipEventDestroy(h->ev_col);
@@ -526,6 +527,13 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     if ((rc = mpc_ws_reserve(h->scales, (size_t)nslot * (n + 128) * (nb / 32)))) return rc;
     if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nt * nb / 32 + 8) * 1024 * sizeof(float)))) return rc;
     if ((rc = mpc_ws_reserve(h->rowstat, (size_t)(n + 128) * sizeof(float)))) return rc;
+    // SYRK work lists: one region per stream that issues trailing updates
+    const int64_t list_ints = mpc_syrk_list_ints(n, MPC_SYRK_TILE);
+    if ((rc = mpc_ws_reserve(h->tlist, sizeof(int) * 3 * (size_t)list_ints))) return rc;
+    auto lists_of = [&](hipStream_t s) {
+        const int r = s == h->pstream ? 0 : (s == h->fstream ? 2 : 1);
+        return (int*)h->tlist.ptr + r * list_ints;
+    };
     mpc_small* sm = (mpc_small*)h->small.ptr;
     uint8_t* payload = (uint8_t*)h->panel.ptr;
     uint8_t* scales = (uint8_t*)h->scales.ptr;
@@ -573,7 +581,8 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         const int q = (int)(j % nslot);
         return mpc_launch_syrk_classes(us, dA + e + e * lda, s, lda, nb, payload + q * pstride,
                                        (int64_t)nb * 4, scales + q * sstride, tr,
-                                       sm->tile_prec[q], sm, cmask, fmask, t0, t1, mixed);
+                                       sm->tile_prec[q], sm, cmask, fmask, t0, t1, mixed,
+                                       lists_of(us));
     };
     // aggregated update of panels p0 .. p0+np-1 on the trailing matrix from
     // block column E (rows and columns >= E nb), tile columns [t0, t1) of
@@ -595,7 +604,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         P.nb = nb;
         P.csrc = (first_src && p0 == 0) ? first_src + e + e * lda : nullptr;
         return mpc_launch_syrk_panels(us, dA + e + e * lda, n - e, lda, nb, P, (int64_t)nb * 4, tr,
-                                      sm, t0, t1, qa, cmask, fmask);
+                                      sm, t0, t1, qa, cmask, fmask, lists_of(us));
     };
 
     const int Bo = mpc_block_bo(n);

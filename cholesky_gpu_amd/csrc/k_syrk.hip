@@ -553,18 +553,57 @@ __device__ __forceinline__ void mm_f32(const operand_src& oc, const operand_src&
 // pair FA/FB) are used, and a tile with none exits without touching C; a
 // tile mixing classes across panels is updated once per class launch.
 // CLS < 0: every panel, the class chosen per workgroup and panel.
+// Work lists of a launch sequence (int32): 16 counters indexed by launch
+// rank (launch_rank_of), then one list of tile pair ids per rank at
+// 16 + rank * cap.  The first launch of a sequence runs over every tile pair
+// and appends each pair to the lists of the later launches that have work on
+// it; those launches then run over their lists only, instead of a full grid
+// of workgroups of which most would find nothing of their class.
+struct syrk_lists {
+    int* base;       // nullptr: every launch runs over the full grid
+    int  cap;        // tile pairs per list
+    int  mode;       // 0 full grid, 1 full grid + build the later lists, 2 from the list
+};
+constexpr int kListHead = 16;
+
 template <int QA, int CLS, int FA, int FB, int KC>
+__device__ __forceinline__ void syrk_tile(float* __restrict__ C, int64_t m, int64_t ldc, int k,
+                                          const mpc_syrk_panels& P, int64_t row_stride,
+                                          int tile_rows, int64_t mt, int64_t tcol0, int64_t pid,
+                                          const syrk_lists& L);
+
+// LISTED: a later launch of a sequence, over its work list (L.mode 2);
+// otherwise one workgroup per tile pair (L.mode 0, or 1 to build the lists)
+template <int QA, int CLS, int FA, int FB, int KC, bool LISTED>
 __global__ __launch_bounds__(256, 2) void syrk_kernel(float* __restrict__ C, int64_t m, int64_t ldc,
                                                    int k, const mpc_syrk_panels P,
                                                    int64_t row_stride, int tile_rows,
                                                    const mpc_small* __restrict__ sm, int64_t npair,
-                                                   int64_t per_xcd, int64_t mt, int64_t tcol0) {
-    constexpr int TILE = 64 * QA;
+                                                   int64_t per_xcd, int64_t mt, int64_t tcol0,
+                                                   const syrk_lists L) {
     if (sm && sm->info != 0) return;
+    if constexpr (LISTED) {
+        const int rank = launch_rank_of(CLS, FA, FB);
+        const int count = L.base[rank];
+        const int* list = L.base + kListHead + (int64_t)rank * L.cap;
+        for (int it = blockIdx.x; it < count; it += gridDim.x)
+            syrk_tile<QA, CLS, FA, FB, KC>(C, m, ldc, k, P, row_stride, tile_rows, mt, tcol0,
+                                           list[it], L);
+        return;
+    }
     // XCD-aware remap: hardware places block b on XCD b % 8
     const int64_t b = blockIdx.x;
     const int64_t pid = (b & 7) * per_xcd + (b >> 3);
     if (pid >= npair) return;
+    syrk_tile<QA, CLS, FA, FB, KC>(C, m, ldc, k, P, row_stride, tile_rows, mt, tcol0, pid, L);
+}
+
+template <int QA, int CLS, int FA, int FB, int KC>
+__device__ __forceinline__ void syrk_tile(float* __restrict__ C, int64_t m, int64_t ldc, int k,
+                                          const mpc_syrk_panels& P, int64_t row_stride,
+                                          int tile_rows, int64_t mt, int64_t tcol0, int64_t pid,
+                                          const syrk_lists& L) {
+    constexpr int TILE = 64 * QA;
     // column-major over the lower triangle from tile column tcol0: column c
     // holds tiles ti = c .. mt-1; start(c) = c*mt - c*(c-1)/2
     auto start = [&](int64_t c) { return c * mt - c * (c - 1) / 2; };
@@ -576,6 +615,22 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(float* __restrict__ C, int
     while (start(tj + 1) <= gid) ++tj;
     const int64_t ti = tj + (gid - start(tj));
     const int64_t trow = (ti * TILE) / tile_rows, tcol = (tj * TILE) / tile_rows;
+
+    if (L.mode == 1 && threadIdx.x == 0) {
+        // the later launches of the sequence with work on this tile pair
+        unsigned need = 0;
+        for (int q = 0; q < P.np; ++q) {
+            const int slot = (P.p0 + q) % P.slots;
+            const uint8_t* prec = P.prec + slot * 1024 + (P.rows0 - (int64_t)q * P.nb) / tile_rows;
+            need |= 1u << launch_rank(prec[trow], prec[tcol]);
+        }
+        need &= ~((2u << launch_rank_of(CLS, FA, FB)) - 1u);
+        for (int r = 0; r < kListHead; ++r)
+            if ((need >> r) & 1u) {
+                const int slot = atomicAdd(&L.base[r], 1);
+                L.base[kListHead + (int64_t)r * L.cap + slot] = (int)pid;
+            }
+    }
 
     const int wave = threadIdx.x >> 6, wr = wave >> 1, wc = wave & 1;
     const bool diag = ti == tj;
@@ -713,8 +768,17 @@ static bool syrk_geometry(int64_t m, int tile, int64_t tcol0, int64_t tcol1, syr
 }
 
 #define MPC_SYRK_LAUNCH(QA, CLS, FA, FB, KC)                                                   \
-    hipLaunchKernelGGL((syrk_kernel<QA, CLS, FA, FB, KC>), g.grid, dim3(256), 0, st, C, m, ldc, \
-                       k, P, row_stride, tile_rows, sm, g.npair, g.per_xcd, g.mt, g.tcol0)
+    do {                                                                                       \
+        const syrk_lists L_ = next_lists();                                                    \
+        if (L_.mode == 2)                                                                      \
+            hipLaunchKernelGGL((syrk_kernel<QA, CLS, FA, FB, KC, true>), list_grid, dim3(256), \
+                               0, st, C, m, ldc, k, P, row_stride, tile_rows, sm, g.npair,     \
+                               g.per_xcd, g.mt, g.tcol0, L_);                                  \
+        else                                                                                   \
+            hipLaunchKernelGGL((syrk_kernel<QA, CLS, FA, FB, KC, false>), g.grid, dim3(256), 0, \
+                               st, C, m, ldc, k, P, row_stride, tile_rows, sm, g.npair,        \
+                               g.per_xcd, g.mt, g.tcol0, L_);                                  \
+    } while (0)
 
 // C(lower) -= Q Q^T restricted to the tile columns [tcol0, tcol1) of C, one
 // panel.  mixed = 0: tiles of MPC_SYRK_TILE, one launch per compute class
@@ -727,7 +791,7 @@ int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, in
                             const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                             int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
                             unsigned class_mask, unsigned fp8_mask, int64_t tcol0, int64_t tcol1,
-                            int mixed) {
+                            int mixed, int* lists) {
     if (mixed != 0 && mixed != 1) {
         mpc_set_error("syrk: mixed must be 0 or 1 (got %d)", mixed);
         return MPC_ERR_ARG;
@@ -740,13 +804,22 @@ int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, in
     P.np = 1;
     P.nb = tile_rows;
     return mpc_launch_syrk_panels(st, C, m, ldc, k, P, row_stride, tile_rows, sm, tcol0, tcol1,
-                                  mixed ? 1 : 2, class_mask, fp8_mask);
+                                  mixed ? 1 : 2, class_mask, fp8_mask, lists);
+}
+
+__global__ void clear_list_heads_kernel(int* __restrict__ heads) {
+    if (threadIdx.x < kListHead) heads[threadIdx.x] = 0;
+}
+
+int64_t mpc_syrk_list_ints(int64_t m, int tile) {
+    const int64_t mt = (m + tile - 1) / tile;
+    return kListHead + 9 * (mt * (mt + 1) / 2);
 }
 
 int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                            const mpc_syrk_panels& P, int64_t row_stride, int tile_rows,
                            const mpc_small* sm, int64_t tcol0, int64_t tcol1, int qa,
-                           unsigned class_mask, unsigned fp8_mask) {
+                           unsigned class_mask, unsigned fp8_mask, int* lists) {
     if (m <= 0) return 0;
     if ((qa != 1 && qa != 2) || P.np < 1 || P.np > MPC_SYRK_MAXP || P.slots < 1 ||
         P.rows0 - (int64_t)(P.np - 1) * P.nb < 0 || P.rows0 % tile_rows || P.nb % tile_rows) {
@@ -758,28 +831,46 @@ int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int
     if (rc) return rc;
     syrk_grid g;
     if (!syrk_geometry(m, 64 * qa, tcol0, tcol1, g)) return 0;
+    syrk_lists none{nullptr, 0, 0};
     if (qa == 1) {                // latency-bound: the K loop unrolled for the usual k
+        const dim3 list_grid = g.grid;
+        auto next_lists = [&]() { return none; };
         if (k == 256) MPC_SYRK_LAUNCH(1, -1, 0, 0, 256);
         else if (k == 128) MPC_SYRK_LAUNCH(1, -1, 0, 0, 128);
         else MPC_SYRK_LAUNCH(1, -1, 0, 0, 0);
         MPC_LAUNCH_CHECK();
         return 0;
     }
-    if (class_mask & (1u << MPC_PREC_E4M3)) {
-        const bool e4 = fp8_mask & 1u, e5 = fp8_mask & 2u;
-        if (e4) MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 0, 0, 0);
-        if (e4 && e5) { MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 0, 1, 0); MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 1, 0, 0); }
-        if (e5) MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 1, 1, 0);
-    }
+    // one launch per compute class (and FP8 element pair) in the masks, in
+    // launch-rank order; with lists, the first one builds the others' lists
+    const bool e4 = (class_mask & (1u << MPC_PREC_E4M3)) && (fp8_mask & 1u);
+    const bool e5 = (class_mask & (1u << MPC_PREC_E4M3)) && (fp8_mask & 2u);
+    const bool f32 = class_mask & (1u << MPC_PREC_FP32);
+    const bool f32s = f32 && (class_mask & ((1u << MPC_PREC_E4M3) | (1u << MPC_PREC_BF16)));
+    const bool f32h = f32 && (class_mask & (1u << MPC_PREC_FP16));
+    const int nlaunch = (int)e4 + 2 * (int)(e4 && e5) + (int)e5 +
+                        (int)(bool)(class_mask & (1u << MPC_PREC_BF16)) +
+                        (int)(bool)(class_mask & (1u << MPC_PREC_FP16)) + (int)f32 + (int)f32s +
+                        (int)f32h;
+    const bool use_lists = lists && nlaunch > 1;
+    int launched = 0;
+    if (use_lists) hipLaunchKernelGGL(clear_list_heads_kernel, dim3(1), dim3(64), 0, st, lists);
+    // later launches: a grid that strides over the list (at most the tile count)
+    const dim3 list_grid((unsigned)std::min<int64_t>(g.per_xcd * 8, 2048));
+    auto next_lists = [&]() {
+        syrk_lists L{use_lists ? lists : nullptr, (int)g.npair,
+                     use_lists ? (launched == 0 ? 1 : 2) : 0};
+        ++launched;
+        return L;
+    };
+    if (e4) MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 0, 0, 0);
+    if (e4 && e5) { MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 0, 1, 0); MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 1, 0, 0); }
+    if (e5) MPC_SYRK_LAUNCH(2, MPC_PREC_E4M3, 1, 1, 0);
     if (class_mask & (1u << MPC_PREC_BF16)) MPC_SYRK_LAUNCH(2, MPC_PREC_BF16, 0, 0, 0);
     if (class_mask & (1u << MPC_PREC_FP16)) MPC_SYRK_LAUNCH(2, MPC_PREC_FP16, 0, 0, 0);
-    if (class_mask & (1u << MPC_PREC_FP32)) {
-        MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 1, 0, 0);          // FP32 x FP32
-        if (class_mask & ((1u << MPC_PREC_E4M3) | (1u << MPC_PREC_BF16)))
-            MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 2, 0, 0);      // FP32 x FP8 / BF16 (split)
-        if (class_mask & (1u << MPC_PREC_FP16))              // FP32 x FP16
-            MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 0, 0, 0);
-    }
+    if (f32) MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 1, 0, 0);         // FP32 x FP32
+    if (f32s) MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 2, 0, 0);        // FP32 x FP8 / BF16 (split)
+    if (f32h) MPC_SYRK_LAUNCH(2, MPC_PREC_FP32, 0, 0, 0);        // FP32 x FP16
     MPC_LAUNCH_CHECK();
     return 0;
 }

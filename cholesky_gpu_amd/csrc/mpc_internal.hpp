@@ -97,6 +97,7 @@ struct mpc_handle {
     mpc_workspace pcg;        // PCG vectors
     mpc_workspace scratch;    // generic
     mpc_workspace winv;       // inverses of the big diagonal blocks of L (Bo x Bo each)
+    mpc_workspace tlist;      // SYRK work lists, one region per factor stream
     mpc_workspace gp64;       // GP: FP64 operator A (n x n)
     mpc_workspace gp32;       // GP: FP32 copy of A (factor source)
     mpc_workspace gpvec;      // GP: right-hand sides, solutions, U, V, partials
@@ -191,15 +192,22 @@ struct mpc_syrk_panels {
 // C(lower) -= sum_p Q_p Q_p^T over tile columns [tcol0, tcol1), tiles of
 // 64 * qa rows: qa = 1 one launch with the class chosen per workgroup and
 // panel; qa = 2 one launch per class in class_mask / fp8 pair in fp8_mask
+// lists (optional, mpc_syrk_list_ints(m, 64 qa) ints on the device, private to
+// the stream): with several class launches, the first builds work lists of
+// the tile pairs each later launch has work on, and those run over their
+// lists only
 int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                            const mpc_syrk_panels& P, int64_t row_stride, int tile_rows,
                            const mpc_small* sm, int64_t tcol0, int64_t tcol1, int qa,
-                           unsigned class_mask = 0xffu, unsigned fp8_mask = 3u);
+                           unsigned class_mask = 0xffu, unsigned fp8_mask = 3u,
+                           int* lists = nullptr);
+int64_t mpc_syrk_list_ints(int64_t m, int tile);
 int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                             const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                             int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
                             unsigned class_mask, unsigned fp8_mask, int64_t tcol0 = 0,
-                            int64_t tcol1 = -1, int mixed = 0);   // mixed: 0 or 1
+                            int64_t tcol1 = -1, int mixed = 0,    // mixed: 0 or 1
+                            int* lists = nullptr);
 // compute classes / FP8 element formats that a factor configuration can produce
 void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* class_mask,
                     unsigned* fp8_mask);
