@@ -75,6 +75,7 @@ __global__ void init_small_kernel(mpc_small* sm) {
         sm->info = 0;
         sm->nan_flag = 0;
         sm->mu_bits = 0x7f800000u;
+        sm->mu_step[0] = sm->mu_step[1] = 0x7f800000u;
     }
     for (int t = threadIdx.x; t < 1024; t += blockDim.x) {
         for (int b = 0; b < 2; ++b) {
@@ -560,7 +561,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         float* Ajj = dA + c0 + c0 * lda;
         int r;
         if ((r = mpc_launch_potrf(ps, Ajj, lda, (int)bs, (int)c0, dinv, sm, ntl,
-                                  sm->tile_amax[b], sm->tile_maxrs[b])))
+                                  sm->tile_amax[b], sm->tile_maxrs[b], b)))
             return r;
         if (s == 0) return 0;
         float* A21 = dA + e + c0 * lda;
@@ -572,7 +573,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                                       sm->tile_maxrs[b], sm, o->update_prec, ladder, o->n_ladder,
                                       eps, sm->tile_prec[q], map ? map + (j + 1) + j * nt : nullptr,
                                       payload + q * pstride, (int64_t)nb * 4, scales + q * sstride,
-                                      &sm->nan_flag);
+                                      &sm->nan_flag, 0, -1, b);
     };
     // trailing update of panel j on stream us restricted to SYRK tile columns
     // [t0, t1) of A22 (tiles of MPC_SYRK_TILE, or of 64 * mixed when mixed)
@@ -726,7 +727,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
             float* A21 = dA + e + c0 * lda;
             MPC_HIP(hipEventRecord(ev[0], st));
             if ((rc = mpc_launch_potrf(st, Ajj, lda, (int)bs, (int)c0, dinv, sm, ntl,
-                                       sm->tile_amax[b], sm->tile_maxrs[b])))
+                                       sm->tile_amax[b], sm->tile_maxrs[b], b)))
                 return rc;
             MPC_HIP(hipEventRecord(ev[1], st));
             if (s > 0) {
@@ -739,7 +740,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                          st, A21, s, nb, lda, tr, nullptr, sm->tile_amax[b], sm->tile_maxrs[b], sm,
                          o->update_prec, ladder, o->n_ladder, eps, sm->tile_prec[b],
                          map ? map + (j + 1) + j * nt : nullptr, payload + b * pstride,
-                         (int64_t)nb * 4, scales + b * sstride, &sm->nan_flag)))
+                         (int64_t)nb * 4, scales + b * sstride, &sm->nan_flag, 0, -1, b)))
                     return rc;
                 MPC_HIP(hipEventRecord(ev[3], st));
                 if ((rc = update(j, 0, -1, st, 0))) return rc;

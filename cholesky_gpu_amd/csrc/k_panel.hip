@@ -45,7 +45,8 @@ __global__ __launch_bounds__(NW * 64, 2) void trsm_stats_kernel(float* __restric
                                                          float* __restrict__ tracker, int tile_rows,
                                                          mpc_small* __restrict__ sm,
                                                          uint32_t* __restrict__ t_amax,
-                                                         uint32_t* __restrict__ t_maxrs) {
+                                                         uint32_t* __restrict__ t_maxrs,
+                                                         int64_t row0) {
     constexpr int NO = NBK / NW;          // blocks per wave
     constexpr int XLD = 33;
     __shared__ float Xs[NBK][32 * XLD];
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(NW * 64, 2) void trsm_stats_kernel(float* __restric
     if (sm->info != 0) return;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = lane & 31, h = lane >> 5;
-    const int64_t r0 = (int64_t)blockIdx.x * 32;
+    const int64_t r0 = row0 + (int64_t)blockIdx.x * 32;
     const int64_t row = r0 + n;
     const bool valid = row < s;
     const int64_t rowc = valid ? row : s - 1;     // loads from clamped rows, then select
@@ -209,9 +210,10 @@ __global__ __launch_bounds__(256) void quantize_kernel(
     const uint8_t* __restrict__ prec_in, const uint32_t* __restrict__ tile_amax,
     const uint32_t* __restrict__ tile_maxrs, const mpc_small* __restrict__ smc, rule_args ra,
     uint8_t* __restrict__ prec_out, uint8_t* __restrict__ map_col, uint8_t* __restrict__ payload,
-    int64_t row_stride, uint8_t* __restrict__ scales, int* __restrict__ nan_flag) {
+    int64_t row_stride, uint8_t* __restrict__ scales, int* __restrict__ nan_flag, int64_t row0,
+    int mu_slot) {
     if (smc && smc->info != 0) return;
-    const int64_t r0 = (int64_t)blockIdx.x * 32;
+    const int64_t r0 = row0 + (int64_t)blockIdx.x * 32;
     const int r = threadIdx.x & 31, qq0 = threadIdx.x >> 5;
     const int64_t row = r0 + r;
     const int tile = (int)(r0 / tile_rows);
@@ -219,7 +221,7 @@ __global__ __launch_bounds__(256) void quantize_kernel(
     if (prec_in) {
         prec = prec_in[tile];
     } else {
-        float mu = __uint_as_float(smc->mu_bits);
+        float mu = __uint_as_float(smc->mu_step[mu_slot]);
         prec = decide(ra, __uint_as_float(tile_maxrs[tile]), mu, __uint_as_float(tile_amax[tile]));
         if (r0 % tile_rows == 0 && threadIdx.x == 0) {
             if (prec_out) prec_out[tile] = (uint8_t)prec;
@@ -295,28 +297,37 @@ __global__ __launch_bounds__(256) void quantize_kernel(
 
 int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, int nb,
                           const float* dinv_j, const float* Ljj, float* rowss, float* tracker,
-                          int tile_rows, mpc_small* sm, uint32_t* tile_amax, uint32_t* tile_maxrs) {
-    if (s <= 0) return 0;
-    if (tile_rows % 32) {
-        mpc_set_error("trsm: tile_rows %d not a multiple of 32", tile_rows);
+                          int tile_rows, mpc_small* sm, uint32_t* tile_amax, uint32_t* tile_maxrs,
+                          int64_t row0, int64_t row1) {
+    if (row1 < 0 || row1 > s) row1 = s;
+    if (row1 <= row0) return 0;
+    if (tile_rows % 32 || row0 % 32) {
+        mpc_set_error("trsm: tile_rows %d / first row %lld not multiples of 32", tile_rows,
+                      (long long)row0);
         return MPC_ERR_ARG;
     }
-    dim3 grid((unsigned)((s + 31) / 32));
+    // rows [row0, row1): the last slab's rows beyond row1 belong to the next
+    // launch only when row1 == s (a slab never straddles row1 otherwise)
+    if (row1 != s && row1 % 32) {
+        mpc_set_error("trsm: row range end %lld not a multiple of 32", (long long)row1);
+        return MPC_ERR_ARG;
+    }
+    dim3 grid((unsigned)((row1 - row0 + 31) / 32));
     switch (nb) {
     case 64:
         hipLaunchKernelGGL((trsm_stats_kernel<2, 2, kTrsmRefine>), grid, dim3(128), 0, st, A21,
                            lda, s, dinv_j, Ljj, rowss, tracker, tile_rows, sm, tile_amax,
-                           tile_maxrs);
+                           tile_maxrs, row0);
         break;
     case 128:
         hipLaunchKernelGGL((trsm_stats_kernel<4, 4, kTrsmRefine>), grid, dim3(256), 0, st, A21,
                            lda, s, dinv_j, Ljj, rowss, tracker, tile_rows, sm, tile_amax,
-                           tile_maxrs);
+                           tile_maxrs, row0);
         break;
     case 256:
         hipLaunchKernelGGL((trsm_stats_kernel<8, 4, kTrsmRefine>), grid, dim3(256), 0, st, A21,
                            lda, s, dinv_j, Ljj, rowss, tracker, tile_rows, sm, tile_amax,
-                           tile_maxrs);
+                           tile_maxrs, row0);
         break;
     default:
         mpc_set_error("trsm: unsupported block size %d", nb);
@@ -341,10 +352,12 @@ int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int
                            const uint32_t* tile_maxrs, mpc_small* sm, int update_prec,
                            const int* ladder, int n_ladder, double eps_prime, uint8_t* prec_out,
                            uint8_t* map_col, uint8_t* payload, int64_t row_stride, uint8_t* scales,
-                           int* nan_flag) {
-    if (m <= 0) return 0;
-    if (k % 32 || tile_rows % 32) {
-        mpc_set_error("quantize: k (%d) and tile_rows (%d) must be multiples of 32", k, tile_rows);
+                           int* nan_flag, int64_t row0, int64_t row1, int mu_slot) {
+    if (row1 < 0 || row1 > m) row1 = m;
+    if (row1 <= row0) return 0;
+    if (k % 32 || tile_rows % 32 || row0 % 32 || (row1 != m && row1 % 32)) {
+        mpc_set_error("quantize: k (%d), tile_rows (%d) and the row range [%lld, %lld) must be "
+                      "multiples of 32", k, tile_rows, (long long)row0, (long long)row1);
         return MPC_ERR_ARG;
     }
     rule_args ra;
@@ -352,9 +365,9 @@ int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int
     ra.n_ladder = n_ladder;
     for (int i = 0; i < 4; ++i) ra.ladder[i] = i < n_ladder ? ladder[i] : 0;
     ra.eps_prime = eps_prime;
-    hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)((m + 31) / 32)), dim3(256), 0, st, P, m, k,
-                       ld, tile_rows, prec_in, tile_amax, tile_maxrs, sm, ra,
-                       prec_out, map_col, payload, row_stride, scales, nan_flag);
+    hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)((row1 - row0 + 31) / 32)), dim3(256), 0, st,
+                       P, m, k, ld, tile_rows, prec_in, tile_amax, tile_maxrs, sm, ra, prec_out,
+                       map_col, payload, row_stride, scales, nan_flag, row0, mu_slot & 1);
     MPC_LAUNCH_CHECK();
     return 0;
 }

@@ -4,14 +4,17 @@
 // Replaces the cusolverDnSpotrf call of switching_chol.cuh:86-95 (CPU form:
 // micro_cholesky, micro_chol.hpp:14-44).  The block lives in LDS as 32x32
 // sub-tiles (lower triangle only, 36 sub-tiles x 32x33 floats = 152 KB for
-// bs = 256); the algorithm is right-looking over 32-column sub-panels:
-//   1. wave 0 factors the diagonal sub-tile (pipelined right-looking, row
+// bs = 256); the algorithm is right-looking over 32-column sub-panels, with
+// only what the next diagonal sub-block needs on wave 0's path:
+//   1. wave 0 factors the diagonal sub-tile kb (pipelined right-looking, row
 //      per lane) and inverts it (column per lane, rows of L broadcast from
 //      LDS);
-//   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32);
-//   3. the trailing sub-tiles receive the rank-32 update (MFMA 32x32x2 f32):
-//      sub-column kb+1 at once, the others while wave 0 factors sub-block
-//      kb+1 (they are disjoint from it).
+//   2. the sub-tiles below are multiplied by that inverse (MFMA 32x32x2 f32):
+//      wave 0 takes sub-tile kb+1 and at once updates diagonal sub-tile
+//      kb+1 with it, the other waves the rest;
+//   3. waves 1..3 apply the rank-32 update to the other trailing sub-tiles
+//      and write sub-column kb back to A while wave 0 factors sub-block kb+1
+//      (they are disjoint from it).  Two workgroup barriers per sub-block.
 // Outputs: L in place (lower; the strict upper triangle of A is untouched),
 // dinv (column-major 32x32 inverse per sub-block), the running minimum pivot
 // squared and, on a non-positive pivot, the 1-based failing column.
@@ -30,8 +33,6 @@ constexpr int NSUB = MAXSB * (MAXSB + 1) / 2;
 // then waits for one).
 constexpr int NW = 4;
 constexpr int NT = 64 * NW;
-// one refinement step after the product with the explicit 32x32 inverse
-constexpr bool kPotrfRefine = false;
 
 typedef float v16f __attribute__((ext_vector_type(16)));
 // LDS (address space 3) pointer: an opaque copy stays a 32-bit LDS address
@@ -232,7 +233,8 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                                                     int col0, float* __restrict__ dinv,
                                                     mpc_small* __restrict__ sm, int ntiles_next,
                                                     uint32_t* __restrict__ st_amax,
-                                                    uint32_t* __restrict__ st_maxrs) {
+                                                    uint32_t* __restrict__ st_maxrs,
+                                                    int mu_slot) {
     __shared__ float T[NSUB * SUBSZ];
     __shared__ float Dinv[SUBSZ];
     __shared__ __align__(16) float colb[2][32];     // column k of L by row parity
@@ -280,22 +282,10 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
     for (int kb = 0; kb < nsb; ++kb) {
         float* D = T + sidx(kb, kb) * SUBSZ;
         // ---- 1. factor + invert the diagonal sub-tile (wave 0); meanwhile
-        // waves 1..3 finish the trailing update of kb-1 on the sub-columns
-        // beyond kb (3b below), which the factor of sub-block kb does not read
+        // waves 1..3 load the rest of the block (kb = 0) or finish the
+        // trailing update of kb-1 (3, below: it does not touch sub-tile kb)
         if (wave != 0 && kb == 0 && vec)                 // the rest of the block
             load_tiles_vec<4>(T, A, lda, bs, nsb, wave - 1, NW - 1);
-        if (wave != 0 && kb >= 1) {
-            const int m = nsb - 1 - kb;                  // sub-columns kb+1 .. nsb-1
-            const int npair = m * (m + 1) / 2;
-            for (int pi = wave - 1; pi < npair; pi += NW - 1) {
-                int ii = 0;
-                while ((ii + 1) * (ii + 2) / 2 <= pi) ++ii;
-                const int jj = pi - ii * (ii + 1) / 2;
-                update_subtile(T, kb + 1 + ii, kb + 1 + jj, kb - 1);
-            }
-            // sub-column kb-1 is final: back to A while the factor runs
-            store_subcol(T, A, lda, bs, kb - 1, 64, NT - 64);
-        }
         if (wave == 0) {
             // Pipelined right-looking factor, row per lane, the row's columns
             // split over the two lane halves by parity: lane r + 32 h holds
@@ -466,14 +456,15 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                 }
             }
         }
-        mpc_lds_barrier();
+        mpc_lds_barrier();          // Dinv(kb) and D(kb); the trailing update of kb-1 done
 
-        // ---- 2. sub-tiles below: X = S * Dinv^T, refined once:
-        //   R = S - X D^T, X += R Dinv^T  (D the factored diagonal sub-tile,
-        //   zero above its diagonal).  Worked in the transposed domain: lane
-        //   (n, h) register i holds X(n, sig(i, h)), the k index of MFMA step t
-        //   is sig(t, h) on both operands, so no LDS round trip is needed.
-        for (int ib = kb + 1 + wave; ib < nsb; ib += NW) {
+        // ---- 2. sub-tiles below: X = S Dinv^T.  Worked in the transposed
+        // domain: lane (n, h) register i holds X(n, sig(i, h)), the k index
+        // of MFMA step t is sig(t, h) on both operands, so no LDS round trip
+        // is needed.  Wave 0 takes the sub-tile that the next diagonal block
+        // depends on (ib = kb+1) and applies it to that block at once
+        // (S(kb+1, kb+1) -= X X^T); the other waves take ib >= kb+2.
+        auto solve_below = [&](int ib) {
             float* S = T + sidx(ib, kb) * SUBSZ;
             const int n = lane & 31, h = lane >> 5;
             v16f s0, x = {};
@@ -487,32 +478,39 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
 #pragma unroll
             for (int t = 0; t < 16; ++t)
                 x = __builtin_amdgcn_mfma_f32_32x32x2f32(dv[t], s0[t], x, 0, 0, 0);
-            if constexpr (kPotrfRefine) {
-                v16f r = s0;
-#pragma unroll
-                for (int t = 0; t < 16; ++t)
-                    r = __builtin_amdgcn_mfma_f32_32x32x2f32(-D[n + sig(t, h) * SUBLD], x[t], r, 0,
-                                                             0, 0);
-#pragma unroll
-                for (int t = 0; t < 16; ++t)
-                    x = __builtin_amdgcn_mfma_f32_32x32x2f32(Dinv[n + sig(t, h) * SUBLD], r[t], x,
-                                                             0, 0, 0);
-            }
 #pragma unroll
             for (int i = 0; i < 16; ++i) S[n + sig(i, h) * SUBLD] = x[i];
+        };
+        if (wave == 0) {
+            if (kb + 1 < nsb) {
+                solve_below(kb + 1);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+                update_subtile(T, kb + 1, kb + 1, kb);
+            }
+        } else {
+            for (int ib = kb + 1 + wave; ib < nsb; ib += NW - 1) solve_below(ib);
         }
-        mpc_lds_barrier();
+        mpc_lds_barrier();          // X(ib, kb) for every ib > kb
 
-        // ---- 3a. trailing update of sub-column kb+1 (what sub-block kb+1
-        // needs next): S(ib,kb+1) -= S(ib,kb) S(kb+1,kb)^T, ib > kb.  The other
-        // sub-columns (3b) are updated during the next factor step.
-        for (int ib = kb + 1 + wave; ib < nsb; ib += NW) update_subtile(T, ib, kb + 1, kb);
-        mpc_lds_barrier();
+        // ---- 3. trailing update by sub-column kb on sub-columns kb+1 ..
+        // (waves 1..3, during the factor of sub-block kb+1, which wave 0
+        // already updated), then sub-column kb, now final, back to A
+        if (wave != 0) {
+            const int m = nsb - 1 - kb;                  // sub-columns kb+1 .. nsb-1
+            const int npair = m * (m + 1) / 2;
+            for (int pi = wave; pi < npair; pi += NW - 1) {   // pair 0 = (kb+1, kb+1): wave 0
+                int ii = 0;
+                while ((ii + 1) * (ii + 2) / 2 <= pi) ++ii;
+                const int jj = pi - ii * (ii + 1) / 2;
+                update_subtile(T, kb + 1 + ii, kb + 1 + jj, kb);
+            }
+            store_subcol(T, A, lda, bs, kb, 64, NT - 64);
+        }
     }
 
-    // ---- write back the last sub-column (the others went back during the
-    // factor), min pivot^2, failure
-    store_subcol(T, A, lda, bs, nsb - 1, 0, NT);
+    // ---- min pivot^2 and failure (every sub-column went back to A in the loop)
     float pmin = INFINITY;
     if (tid < bs) {
         const float d = T[sidx(tid >> 5, tid >> 5) * SUBSZ + (tid & 31) * (1 + SUBLD)];
@@ -529,7 +527,9 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
             sm->info = col0 + fail_col + 1;
             if (fail_nan) atomicOr(&sm->nan_flag, 1);
         } else {
-            atomicMin(&sm->mu_bits, __float_as_uint(red[0]));
+            const uint32_t mine = __float_as_uint(red[0]);
+            const uint32_t before = atomicMin(&sm->mu_bits, mine);
+            sm->mu_step[mu_slot] = before < mine ? before : mine;
         }
     }
 }
@@ -537,7 +537,8 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
 }  // namespace
 
 int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0, float* dinv,
-                     mpc_small* sm, int ntiles_next, uint32_t* stats_amax, uint32_t* stats_maxrs) {
+                     mpc_small* sm, int ntiles_next, uint32_t* stats_amax, uint32_t* stats_maxrs,
+                     int mu_slot) {
     if (bs <= 0 || bs > 256) {
         mpc_set_error("potrf: block size %d outside 1..256", bs);
         return MPC_ERR_ARG;
@@ -550,8 +551,12 @@ int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0, fl
         mpc_set_error("potrf: %d panel tiles exceed 1024", ntiles_next);
         return MPC_ERR_ARG;
     }
+    if (mu_slot != 0 && mu_slot != 1) {
+        mpc_set_error("potrf: mu slot %d", mu_slot);
+        return MPC_ERR_ARG;
+    }
     hipLaunchKernelGGL(potrf_kernel, dim3(1), dim3(NT), 0, st, A, lda, bs, col0, dinv, sm,
-                       ntiles_next, stats_amax, stats_maxrs);
+                       ntiles_next, stats_amax, stats_maxrs, mu_slot);
     MPC_LAUNCH_CHECK();
     return 0;
 }

@@ -566,10 +566,18 @@ struct syrk_lists {
 };
 constexpr int kListHead = 16;
 
+// The tile pairs of a launch: the lower triangle of tile columns
+// [tcol0, tcol0 + ...) (column-major), or, when rect_ncol > 0, the rectangle
+// of tile rows [rect_tr0, mt) and tile columns [tcol0, tcol0 + rect_ncol)
+// (row-major; every pair strictly below the diagonal).
+struct syrk_shape {
+    int64_t mt, tcol0, rect_tr0, rect_ncol;
+};
+
 template <int QA, int CLS, int FA, int FB, int KC>
 __device__ __forceinline__ void syrk_tile(float* __restrict__ C, int64_t m, int64_t ldc, int k,
                                           const mpc_syrk_panels& P, int64_t row_stride,
-                                          int tile_rows, int64_t mt, int64_t tcol0, int64_t pid,
+                                          int tile_rows, const syrk_shape& G, int64_t pid,
                                           const syrk_lists& L);
 
 // LISTED: a later launch of a sequence, over its work list (L.mode 2);
@@ -579,7 +587,7 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(float* __restrict__ C, int
                                                    int k, const mpc_syrk_panels P,
                                                    int64_t row_stride, int tile_rows,
                                                    const mpc_small* __restrict__ sm, int64_t npair,
-                                                   int64_t per_xcd, int64_t mt, int64_t tcol0,
+                                                   int64_t per_xcd, const syrk_shape G,
                                                    const syrk_lists L) {
     if (sm && sm->info != 0) return;
     if constexpr (LISTED) {
@@ -587,33 +595,39 @@ __global__ __launch_bounds__(256, 2) void syrk_kernel(float* __restrict__ C, int
         const int count = L.base[rank];
         const int* list = L.base + kListHead + (int64_t)rank * L.cap;
         for (int it = blockIdx.x; it < count; it += gridDim.x)
-            syrk_tile<QA, CLS, FA, FB, KC>(C, m, ldc, k, P, row_stride, tile_rows, mt, tcol0,
-                                           list[it], L);
+            syrk_tile<QA, CLS, FA, FB, KC>(C, m, ldc, k, P, row_stride, tile_rows, G, list[it], L);
         return;
     }
     // XCD-aware remap: hardware places block b on XCD b % 8
     const int64_t b = blockIdx.x;
     const int64_t pid = (b & 7) * per_xcd + (b >> 3);
     if (pid >= npair) return;
-    syrk_tile<QA, CLS, FA, FB, KC>(C, m, ldc, k, P, row_stride, tile_rows, mt, tcol0, pid, L);
+    syrk_tile<QA, CLS, FA, FB, KC>(C, m, ldc, k, P, row_stride, tile_rows, G, pid, L);
 }
 
 template <int QA, int CLS, int FA, int FB, int KC>
 __device__ __forceinline__ void syrk_tile(float* __restrict__ C, int64_t m, int64_t ldc, int k,
                                           const mpc_syrk_panels& P, int64_t row_stride,
-                                          int tile_rows, int64_t mt, int64_t tcol0, int64_t pid,
+                                          int tile_rows, const syrk_shape& G, int64_t pid,
                                           const syrk_lists& L) {
     constexpr int TILE = 64 * QA;
-    // column-major over the lower triangle from tile column tcol0: column c
-    // holds tiles ti = c .. mt-1; start(c) = c*mt - c*(c-1)/2
-    auto start = [&](int64_t c) { return c * mt - c * (c - 1) / 2; };
-    const int64_t gid = pid + start(tcol0);
-    const double tw = 2.0 * (double)mt + 1.0;
-    int64_t tj = (int64_t)((tw - sqrt(tw * tw - 8.0 * (double)gid)) * 0.5);
-    if (tj < 0) tj = 0;
-    while (tj > 0 && start(tj) > gid) --tj;
-    while (start(tj + 1) <= gid) ++tj;
-    const int64_t ti = tj + (gid - start(tj));
+    int64_t ti, tj;
+    if (G.rect_ncol > 0) {
+        ti = G.rect_tr0 + pid / G.rect_ncol;
+        tj = G.tcol0 + pid % G.rect_ncol;
+    } else {
+        // column-major over the lower triangle from tile column tcol0: column c
+        // holds tiles ti = c .. mt-1; start(c) = c*mt - c*(c-1)/2
+        const int64_t mt = G.mt;
+        auto start = [&](int64_t c) { return c * mt - c * (c - 1) / 2; };
+        const int64_t gid = pid + start(G.tcol0);
+        const double tw = 2.0 * (double)mt + 1.0;
+        tj = (int64_t)((tw - sqrt(tw * tw - 8.0 * (double)gid)) * 0.5);
+        if (tj < 0) tj = 0;
+        while (tj > 0 && start(tj) > gid) --tj;
+        while (start(tj + 1) <= gid) ++tj;
+        ti = tj + (gid - start(tj));
+    }
     const int64_t trow = (ti * TILE) / tile_rows, tcol = (tj * TILE) / tile_rows;
 
     if (L.mode == 1 && threadIdx.x == 0) {
@@ -753,14 +767,25 @@ static int syrk_check(int64_t ldc, int k, int tile_rows) {
 // launch geometry over tile columns [tcol0, tcol1) of the lower triangle
 struct syrk_grid {
     int64_t mt, npair, per_xcd, tcol0;
+    syrk_shape shape;
     dim3 grid;
 };
-static bool syrk_geometry(int64_t m, int tile, int64_t tcol0, int64_t tcol1, syrk_grid& g) {
+// trow0 > 0: the rectangle of tile rows [trow0, mt) x tile columns [tcol0,
+// tcol1) (trow0 >= tcol1: strictly below the diagonal)
+static bool syrk_geometry(int64_t m, int tile, int64_t tcol0, int64_t tcol1, syrk_grid& g,
+                          int64_t trow0 = 0) {
     g.mt = (m + tile - 1) / tile;
     if (tcol1 < 0 || tcol1 > g.mt) tcol1 = g.mt;
     if (tcol0 >= tcol1) return false;
-    auto start = [&](int64_t c) { return c * g.mt - c * (c - 1) / 2; };
-    g.npair = start(tcol1) - start(tcol0);
+    if (trow0 > 0) {
+        if (trow0 >= g.mt) return false;
+        g.npair = (g.mt - trow0) * (tcol1 - tcol0);
+        g.shape = syrk_shape{g.mt, tcol0, trow0, tcol1 - tcol0};
+    } else {
+        auto start = [&](int64_t c) { return c * g.mt - c * (c - 1) / 2; };
+        g.npair = start(tcol1) - start(tcol0);
+        g.shape = syrk_shape{g.mt, tcol0, 0, 0};
+    }
     g.per_xcd = (g.npair + 7) / 8;
     g.tcol0 = tcol0;
     g.grid = dim3((unsigned)(g.per_xcd * 8));
@@ -773,11 +798,11 @@ static bool syrk_geometry(int64_t m, int tile, int64_t tcol0, int64_t tcol1, syr
         if (L_.mode == 2)                                                                      \
             hipLaunchKernelGGL((syrk_kernel<QA, CLS, FA, FB, KC, true>), list_grid, dim3(256), \
                                0, st, C, m, ldc, k, P, row_stride, tile_rows, sm, g.npair,     \
-                               g.per_xcd, g.mt, g.tcol0, L_);                                  \
+                               g.per_xcd, g.shape, L_);                                        \
         else                                                                                   \
             hipLaunchKernelGGL((syrk_kernel<QA, CLS, FA, FB, KC, false>), g.grid, dim3(256), 0, \
                                st, C, m, ldc, k, P, row_stride, tile_rows, sm, g.npair,        \
-                               g.per_xcd, g.mt, g.tcol0, L_);                                  \
+                               g.per_xcd, g.shape, L_);                                        \
     } while (0)
 
 // C(lower) -= Q Q^T restricted to the tile columns [tcol0, tcol1) of C, one
@@ -791,7 +816,7 @@ int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, in
                             const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                             int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
                             unsigned class_mask, unsigned fp8_mask, int64_t tcol0, int64_t tcol1,
-                            int mixed, int* lists) {
+                            int mixed, int* lists, int64_t trow0) {
     if (mixed != 0 && mixed != 1) {
         mpc_set_error("syrk: mixed must be 0 or 1 (got %d)", mixed);
         return MPC_ERR_ARG;
@@ -804,7 +829,7 @@ int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, in
     P.np = 1;
     P.nb = tile_rows;
     return mpc_launch_syrk_panels(st, C, m, ldc, k, P, row_stride, tile_rows, sm, tcol0, tcol1,
-                                  mixed ? 1 : 2, class_mask, fp8_mask, lists);
+                                  mixed ? 1 : 2, class_mask, fp8_mask, lists, trow0);
 }
 
 __global__ void clear_list_heads_kernel(int* __restrict__ heads) {
@@ -819,7 +844,7 @@ int64_t mpc_syrk_list_ints(int64_t m, int tile) {
 int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                            const mpc_syrk_panels& P, int64_t row_stride, int tile_rows,
                            const mpc_small* sm, int64_t tcol0, int64_t tcol1, int qa,
-                           unsigned class_mask, unsigned fp8_mask, int* lists) {
+                           unsigned class_mask, unsigned fp8_mask, int* lists, int64_t trow0) {
     if (m <= 0) return 0;
     if ((qa != 1 && qa != 2) || P.np < 1 || P.np > MPC_SYRK_MAXP || P.slots < 1 ||
         P.rows0 - (int64_t)(P.np - 1) * P.nb < 0 || P.rows0 % tile_rows || P.nb % tile_rows) {
@@ -830,7 +855,12 @@ int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int
     int rc = syrk_check(ldc, k, tile_rows);
     if (rc) return rc;
     syrk_grid g;
-    if (!syrk_geometry(m, 64 * qa, tcol0, tcol1, g)) return 0;
+    if (trow0 > 0 && tcol1 >= 0 && trow0 < tcol1) {
+        mpc_set_error("syrk: rectangle rows from tile %lld overlap its columns up to %lld",
+                      (long long)trow0, (long long)tcol1);
+        return MPC_ERR_ARG;
+    }
+    if (!syrk_geometry(m, 64 * qa, tcol0, tcol1, g, trow0)) return 0;
     syrk_lists none{nullptr, 0, 0};
     if (qa == 1) {                // latency-bound: the K loop unrolled for the usual k
         const dim3 list_grid = g.grid;

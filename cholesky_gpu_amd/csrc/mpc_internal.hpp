@@ -134,6 +134,10 @@ struct mpc_small {
     int      info;            // 1-based failing column, 0 = ok
     int      nan_flag;        // NaN/Inf seen in a panel
     uint32_t mu_bits;         // min pivot^2 so far (bits of a positive float; atomicMin)
+    // mu_bits as the POTRF of a step left it, by step parity: the tile
+    // decisions of that step's panel read this copy, which stays fixed while
+    // the next step's POTRF lowers mu_bits
+    uint32_t mu_step[2];
     uint32_t pad;
     // per-step tile stats (index by tile within the panel), max 1024 tiles,
     // double-buffered by step parity so panel j+1 can be prepared while the
@@ -152,17 +156,20 @@ static_assert(sizeof(((mpc_small*)nullptr)->tile_prec) / 1024 == MPC_PANEL_SLOTS
 // k_potrf.hip
 int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0,
                      float* dinv, mpc_small* sm, int ntiles_next, uint32_t* stats_amax,
-                     uint32_t* stats_maxrs);
+                     uint32_t* stats_maxrs, int mu_slot = 0);
 // k_panel.hip
+// rows [row0, row1) of the s-row panel (row1 < 0: s); the panel kernels
+// index tiles and statistics by the row within the whole panel
 int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, int nb,
                           const float* dinv_j, const float* Ljj, float* rowss, float* tracker,
-                          int tile_rows, mpc_small* sm, uint32_t* tile_amax, uint32_t* tile_maxrs);
+                          int tile_rows, mpc_small* sm, uint32_t* tile_amax, uint32_t* tile_maxrs,
+                          int64_t row0 = 0, int64_t row1 = -1);
 int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
                            int tile_rows, const uint8_t* prec_in, const uint32_t* tile_amax,
                            const uint32_t* tile_maxrs, mpc_small* sm, int update_prec,
                            const int* ladder, int n_ladder, double eps_prime, uint8_t* prec_out,
                            uint8_t* map_col, uint8_t* payload, int64_t row_stride, uint8_t* scales,
-                           int* nan_flag);
+                           int* nan_flag, int64_t row0 = 0, int64_t row1 = -1, int mu_slot = 0);
 int mpc_launch_tile_amax(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
                          int tile_rows, uint32_t* tile_amax);
 // k_syrk.hip
@@ -191,7 +198,9 @@ struct mpc_syrk_panels {
 };
 // C(lower) -= sum_p Q_p Q_p^T over tile columns [tcol0, tcol1), tiles of
 // 64 * qa rows: qa = 1 one launch with the class chosen per workgroup and
-// panel; qa = 2 one launch per class in class_mask / fp8 pair in fp8_mask
+// panel; qa = 2 one launch per class in class_mask / fp8 pair in fp8_mask.
+// trow0 > 0: only the tile rows from trow0 (>= tcol1: a rectangle below the
+// diagonal)
 // lists (optional, mpc_syrk_list_ints(m, 64 qa) ints on the device, private to
 // the stream): with several class launches, the first builds work lists of
 // the tile pairs each later launch has work on, and those run over their
@@ -200,14 +209,14 @@ int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int
                            const mpc_syrk_panels& P, int64_t row_stride, int tile_rows,
                            const mpc_small* sm, int64_t tcol0, int64_t tcol1, int qa,
                            unsigned class_mask = 0xffu, unsigned fp8_mask = 3u,
-                           int* lists = nullptr);
+                           int* lists = nullptr, int64_t trow0 = 0);
 int64_t mpc_syrk_list_ints(int64_t m, int tile);
 int mpc_launch_syrk_classes(hipStream_t st, float* C, int64_t m, int64_t ldc, int k,
                             const uint8_t* payload, int64_t row_stride, const uint8_t* scales,
                             int tile_rows, const uint8_t* prec_of_tile, const mpc_small* sm,
                             unsigned class_mask, unsigned fp8_mask, int64_t tcol0 = 0,
                             int64_t tcol1 = -1, int mixed = 0,    // mixed: 0 or 1
-                            int* lists = nullptr);
+                            int* lists = nullptr, int64_t trow0 = 0);
 // compute classes / FP8 element formats that a factor configuration can produce
 void mpc_syrk_masks(int update_prec, const int* ladder, int n_ladder, unsigned* class_mask,
                     unsigned* fp8_mask);
