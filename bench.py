@@ -42,7 +42,11 @@ def main():
                     help="per-tile precision ladder of the mixed run, lowest first")
     ap.add_argument("--syrk-reps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="time only the first CPU_STEPS block steps of the CPU factor and "
+                         "extrapolate by flops (0: the whole factor and PCG)")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP64 forward/backward error report")
     ap.add_argument("--syrk-only", action="store_true",
                     help="only the trailing update at the first-step shape and a 1 GiB device "
                          "copy of known size (for hardware-counter runs), one JSON line")
@@ -65,8 +69,10 @@ def main():
 
     if args.syrk_only:
         sy = syrk_roofline(h, stream, n, nb, args.syrk_reps, C0=None)
+        sya = syrk_roofline(h, stream, n, nb, args.syrk_reps, C0=None,
+                            k=(args.agg or 2) * nb)
         cp = copy_reference(stream, args.syrk_reps)
-        print(json.dumps({"syrk": sy, "copy": cp}), flush=True)
+        print(json.dumps({"syrk": sy, "syrk_aggregated": sya, "copy": cp}), flush=True)
         h.close()
         return
 
@@ -112,9 +118,32 @@ def main():
     del A64
     log(f"fp8: {ms8:.2f} ms its={pi8.iters} rr={rr8:.2e}; fp32: {ms32:.2f} ms its={pi32.iters} rr={rr32:.2e}")
 
-    # ---- SYRK roofline at the first-step shape (m = n - nb, k = nb), E4M3 tiles
+    # ---- accuracy (outside the timed region): forward error against an FP64
+    # host Cholesky solve of the same FP32 matrix, backward error of each
+    # factor (the same factor the solve used: factored again into a buffer of
+    # its own with the same options), Lanczos estimate of cond(M^-1 A)
+    acc = None
+    if not args.no_accuracy:
+        from cholesky_gpu_amd import accuracy as ACC
+        acc = {}
+        Xs, where = ACC.fp64_solution(A, B)
+        for name, fo, X, pi in (("fp8", fo8, X8, pi8), ("fp32", fo32, X32, pi32)):
+            Lf = A.clone()
+            rcf, _, _, _ = S.factor(h, Lf, fo, want_map=False)
+            acc[name] = {"fwd_err": ACC.forward_error(X, Xs),
+                         "bwd_err": ACC.backward_error(A, Lf) if rcf == 0 else None,
+                         "kappa_est": round(float(pi.kappa_est), 4)}
+            del Lf
+        acc["fwd_ref"] = f"FP64 Cholesky of the FP32 matrix ({where})"
+        del Xs
+        log("accuracy", acc)
+
+    # ---- SYRK roofline at the first-step shape (m = n - nb, k = nb), E4M3 tiles,
+    # and at the aggregated shape the factor issues (k = agg_panels * nb)
     sy = syrk_roofline(h, stream, n, nb, args.syrk_reps, C0=A)
     m, k, syrk_ms, syrk_gbs, syrk_tf = sy["m"], sy["k"], sy["ms"], sy["gbs"], sy["tflops"]
+    agg = args.agg if args.agg is not None else fo8.agg_panels
+    sya = syrk_roofline(h, stream, n, nb, args.syrk_reps, C0=A, k=agg * nb) if agg > 1 else None
 
     # ---- CPU restatement on a bounded sample (rank 0, N = 1)
     cpu = None
@@ -123,6 +152,15 @@ def main():
             cpu = cpu_baseline(n, nb, args.eps, pi8.iters, args.cpu_steps, fi8, h, A, B)
         except Exception as e:  # the GPU result stands without the CPU estimate
             log("cpu baseline failed:", repr(e))
+
+    # PCG bytes per iteration: the upper triangle of A (SYMV) and the lower
+    # triangle of L twice (two sweeps), FP32; the start (x0 = M^-1 b, r0,
+    # z0) is one more product and two more preconditioner applications
+    tri = n * (n + 1) / 2 * 4.0
+    def pcg_rate(pi, ms):
+        nbytes = (pi.iters * 3 + 5) * tri
+        return round(nbytes / (ms * 1e-3) / 1e9, 1)
+    pcg_gbs = {"fp8": pcg_rate(pi8, pi8.ms_total), "fp32": pcg_rate(pi32, pi32.ms_total)}
 
     tot_flops = max(1.0, sum(fi8.syrk_flops))
     flop_frac = {L.PREC_NAMES[p]: round(fi8.syrk_flops[p] / tot_flops, 4)
@@ -154,12 +192,28 @@ def main():
         "syrk_flop_fraction": flop_frac,
         "factor_tflops": {"fp8": round(fi8.flops_total / (fi8.ms_total * 1e-3) / 1e12, 2),
                           "fp32": round(fi32.flops_total / (fi32.ms_total * 1e-3) / 1e12, 2)},
+        "factor_frac_peak": {
+            "fp8_vs_fp8_peak": round(fi8.flops_total / (fi8.ms_total * 1e-3) / 1e12
+                                     / FP8_PEAK_TFLOPS, 4),
+            "fp32_vs_fp32_mfma_peak": round(fi32.flops_total / (fi32.ms_total * 1e-3) / 1e12
+                                            / FP32_MFMA_PEAK_TFLOPS, 4)},
+        "pcg_gbs": pcg_gbs,
+        "pcg_frac_hbm": {kk: round(v / HBM_PEAK_GBS, 4) for kk, v in pcg_gbs.items()},
+        "accuracy": acc,
         "roofline": {"kernel": "syrk_kernel<E4M3> (one trailing update, m=%d k=%d)" % (m, k),
                      "bound": "hbm", "achieved": round(syrk_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(syrk_gbs / HBM_PEAK_GBS, 4),
                      "traffic": syrk_traffic(m, k),
                      "avg_ms": round(syrk_ms, 4), "tflops": round(syrk_tf, 1),
                      "mfma_frac_fp8": round(syrk_tf / FP8_PEAK_TFLOPS, 4)},
+        "roofline_aggregated": None if sya is None else {
+            "kernel": "syrk_kernel<E4M3> (the factor's aggregated update, m=%d k=%d)"
+                      % (sya["m"], sya["k"]),
+            "bound": "hbm", "achieved": round(sya["gbs"], 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(sya["gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": syrk_traffic(sya["m"], sya["k"]), "avg_ms": round(sya["ms"], 4),
+            "tflops": round(sya["tflops"], 1),
+            "mfma_frac_fp8": round(sya["tflops"] / FP8_PEAK_TFLOPS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -171,28 +225,30 @@ def syrk_traffic(m, k):
     committed FETCH_SIZE / WRITE_SIZE counter runs (each scaled by what the
     same counter reads for a copy kernel of known size); None when no counter
     run of this shape is on file."""
-    path = os.path.join(ROOT, "profiles", "syrk_e4m3_n16384_step0_counters.json")
-    try:
-        with open(path) as f:
-            rec = json.load(f)
-    except (OSError, ValueError):
-        return None
-    if rec.get("m") != m or rec.get("k") != k:
-        return None
-    return int(round(rec["traffic_bytes"]))
+    for name in ("syrk_e4m3_n16384_step0_counters.json", "syrk_e4m3_n16384_agg_counters.json"):
+        try:
+            with open(os.path.join(ROOT, "profiles", name)) as f:
+                rec = json.load(f)
+        except (OSError, ValueError):
+            continue
+        if rec.get("m") == m and rec.get("k") == k:
+            return int(round(rec["traffic_bytes"]))
+    return None
 
 
-def syrk_roofline(h, stream, n, nb, reps, C0=None):
-    """One trailing update at the first-step shape (m = n - nb, k = nb) with
-    all panel tiles in E4M3, timed with HIP events on the stream the kernel
-    runs on.  Algorithmic bytes: the lower triangle of C read and written in
-    FP32 (8 B per element incl. the diagonal), the E4M3 panel and its scales."""
+def syrk_roofline(h, stream, n, nb, reps, C0=None, k=None):
+    """One trailing update at the first-step shape (m = n - nb, k = nb, or the
+    aggregated k = a nb) with all panel tiles in E4M3, timed with HIP events
+    on the stream the kernel runs on.  Algorithmic bytes: the lower triangle
+    of C read and written in FP32 (8 B per element incl. the diagonal), the
+    E4M3 panel and its scales."""
     import ctypes as C
     import numpy as np
     import torch
     from cholesky_gpu_amd import _lib as L
     from cholesky_gpu_amd import solver as S
-    m, k, tr = n - nb, nb, nb
+    m, tr = n - nb, nb
+    k = k or nb
     rng = np.random.default_rng(1)
     P = torch.from_numpy(rng.standard_normal((k, m)).astype(np.float32)).cuda()
     ntile = (m + tr - 1) // tr
@@ -254,52 +310,65 @@ def copy_reference(stream, reps, nbytes=1 << 30):
 
 def cpu_baseline(n, nb, eps, gpu_iters, cpu_steps, fi8, h, A, B):
     """The oracle's restatement of the reference path (FP32 micro_cholesky
-    diagonal blocks, TRSM, quantised FP32-accumulate SYRK, FP64 PCG) timed on
-    the host: the first cpu_steps block steps of the factor plus two PCG
-    solves with maxit 1 and 3, extrapolated by flop count / iterations."""
+    diagonal blocks, TRSM, quantised FP32-accumulate SYRK with the same
+    per-tile switching, FP64 PCG) timed on the host's cores: by default the
+    whole factor of the same n = 16384 matrix and the PCG to the same 1e-10
+    true residual on that CPU factor, plus the small RBF workload (2048
+    points, block 128, FP16 update, PCG to 1e-8) in full.  cpu_steps > 0
+    times only the first block steps and extrapolates by flops."""
     import numpy as np
     from oracle import oracle as O
     log("cpu baseline: generating A on the host")
     Ah, _ = O.householder_spd(n, 1e6, 1e6, SEED_A)
-    steps = min(cpu_steps, (n + nb - 1) // nb)
-    t0 = time.perf_counter()
-    Lh, rc, mp, ud, fi = O.factor_mixed(Ah, nb=nb, update_prec=-1, ladder=(O.E4M3, O.FP32),
-                                        eps_prime=eps, max_steps=steps)
-    t_steps = time.perf_counter() - t0
-    # flops of the sampled steps vs the whole factor (trailing update + panel)
-    def step_flops(j):
-        s = n - (j + 1) * nb
-        return s * (s + 1.0) * nb + s * nb * nb + nb ** 3 / 3.0
     nt = (n + nb - 1) // nb
-    f_sample = sum(step_flops(j) for j in range(steps))
-    f_total = sum(step_flops(j) for j in range(nt))
-    t_factor = t_steps * f_total / f_sample
-    del Lh
-    # PCG iteration cost with the FP32-update factor computed on the GPU (same L shape)
-    import torch
-    Lg = torch.empty_like(A)
-    Lg.copy_(A)
-    from cholesky_gpu_amd import solver as S
-    from cholesky_gpu_amd import _lib as L
-    S.factor(h, Lg, S.factor_opts(nb=nb, update_prec=L.PREC_FP32, ladder=(L.PREC_FP32,)))
-    Lnp = np.asfortranarray(np.tril(Lg.cpu().numpy().T))
-    del Lg
     b = B.cpu().numpy()[0]
-    t1 = time.perf_counter()
-    O.pcg(Ah, Lnp, b, tol=1e-300, maxit=1)
-    t_a = time.perf_counter() - t1
-    t1 = time.perf_counter()
-    O.pcg(Ah, Lnp, b, tol=1e-300, maxit=3)
-    t_b = time.perf_counter() - t1
-    t_it = max(1e-6, (t_b - t_a) / 2.0)
-    t_setup = max(0.0, t_a - t_it)
-    total_s = t_factor + t_setup + gpu_iters * t_it
-    log(f"cpu: {steps} steps {t_steps:.2f}s -> factor {t_factor:.1f}s; pcg it {t_it:.3f}s")
+    if cpu_steps <= 0:
+        t0 = time.perf_counter()
+        Lh, rc, mp, ud, fi = O.factor_mixed(Ah, nb=nb, update_prec=-1, ladder=(O.E4M3, O.FP32),
+                                            eps_prime=eps)
+        t_factor = time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"CPU factor failed (info {rc})")
+        t1 = time.perf_counter()
+        _, it_c, rr_c, _ = O.pcg(Ah, np.tril(Lh), b, tol=1e-10, maxit=500)
+        t_pcg = time.perf_counter() - t1
+        del Lh
+        total_s = t_factor + t_pcg
+        sample = (f"full: factor of the n={n} matrix ({t_factor:.1f} s) + PCG on that factor to "
+                  f"a 1e-10 FP64 true residual ({it_c} iterations, {t_pcg:.1f} s, relres "
+                  f"{rr_c:.1e})")
+        log(f"cpu: factor {t_factor:.1f}s, pcg {t_pcg:.1f}s ({it_c} its)")
+    else:
+        steps = min(cpu_steps, nt)
+        t0 = time.perf_counter()
+        Lh, rc, mp, ud, fi = O.factor_mixed(Ah, nb=nb, update_prec=-1, ladder=(O.E4M3, O.FP32),
+                                            eps_prime=eps, max_steps=steps)
+        t_steps = time.perf_counter() - t0
+        del Lh
+
+        def step_flops(j):
+            s = n - (j + 1) * nb
+            return s * (s + 1.0) * nb + s * nb * nb + nb ** 3 / 3.0
+        t_factor = t_steps * sum(step_flops(j) for j in range(nt)) / sum(
+            step_flops(j) for j in range(steps))
+        total_s = t_factor
+        sample = (f"first {steps} of {nt} block steps of the factor ({t_steps:.1f} s), "
+                  f"extrapolated by flops; PCG not timed")
+    del Ah
+    # the small RBF workload in full
+    t2 = time.perf_counter()
+    Xp = O.points_gaussian(2048, 16, 11)
+    K = np.asarray(O.rbf_matrix(Xp, [4.0], 1.0, 1e-2), np.float32)
+    b1 = np.random.default_rng(1).standard_normal(2048)
+    L1, rc1, _, _, _ = O.factor_mixed(K, nb=128, update_prec=O.FP16, ladder=(O.FP16,),
+                                      tile_rows=128)
+    _, it1, rr1, _ = O.pcg(K, np.tril(L1), b1, tol=1e-8, maxit=200)
+    t_c1 = time.perf_counter() - t2
     return {"value": round(total_s * 1e3, 1), "unit": "ms", "cores": O.lib().orc_num_threads(),
-            "kind": "port",
-            "sample": f"first {steps} of {nt} block steps of the factor ({t_steps:.1f} s, "
-                      f"extrapolated by flops) + PCG iteration cost from maxit 1 vs 3 "
-                      f"({t_it:.3f} s/it) x {gpu_iters} iterations; n={n}"}
+            "kind": "port", "sample": sample,
+            "rbf_2048_fp16_ms": round(t_c1 * 1e3, 1),
+            "rbf_2048_fp16_sample": f"full: 2048-point RBF matrix, block-128 FP16-update factor "
+                                    f"and PCG to 1e-8 ({it1} iterations), generation included"}
 
 
 if __name__ == "__main__":

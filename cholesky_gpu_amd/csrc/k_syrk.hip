@@ -571,7 +571,7 @@ constexpr int kListHead = 16;
 // of tile rows [rect_tr0, mt) and tile columns [tcol0, tcol0 + rect_ncol)
 // (row-major; every pair strictly below the diagonal).
 struct syrk_shape {
-    int64_t mt, tcol0, rect_tr0, rect_ncol;
+    int64_t mt, tcol0, tcol1, rect_tr0, rect_ncol;
 };
 
 template <int QA, int CLS, int FA, int FB, int KC>
@@ -616,17 +616,53 @@ __device__ __forceinline__ void syrk_tile(float* __restrict__ C, int64_t m, int6
         ti = G.rect_tr0 + pid / G.rect_ncol;
         tj = G.tcol0 + pid % G.rect_ncol;
     } else {
-        // column-major over the lower triangle from tile column tcol0: column c
-        // holds tiles ti = c .. mt-1; start(c) = c*mt - c*(c-1)/2
+        // The lower triangle from tile column tcol0 in super-blocks of SB x SB
+        // tiles (2048 rows and columns): super-columns in order, in each the
+        // diagonal super-block and then the super-blocks below it, each
+        // column-major.  Consecutive pair ids go to one XCD, whose L2 then
+        // keeps the panel rows of the few super-blocks it works on (a plain
+        // column-major order streams the whole panel past every tile column).
+        // start(c) = c*mt - c*(c-1)/2 counts the pairs left of tile column c.
+        constexpr int64_t SB = 2048 / TILE;
         const int64_t mt = G.mt;
         auto start = [&](int64_t c) { return c * mt - c * (c - 1) / 2; };
         const int64_t gid = pid + start(G.tcol0);
         const double tw = 2.0 * (double)mt + 1.0;
-        tj = (int64_t)((tw - sqrt(tw * tw - 8.0 * (double)gid)) * 0.5);
-        if (tj < 0) tj = 0;
-        while (tj > 0 && start(tj) > gid) --tj;
-        while (start(tj + 1) <= gid) ++tj;
-        ti = tj + (gid - start(tj));
+        int64_t tc = (int64_t)((tw - sqrt(tw * tw - 8.0 * (double)gid)) * 0.5);
+        if (tc < 0) tc = 0;
+        while (tc > 0 && start(tc) > gid) --tc;
+        while (start(tc + 1) <= gid) ++tc;
+        // the super-column holding tile column tc: columns [c0, c1)
+        const int64_t c0 = G.tcol0 + ((tc - G.tcol0) / SB) * SB;
+        const int64_t c1 = c0 + SB < G.tcol1 ? c0 + SB : G.tcol1;
+        const int64_t rb = c0 + SB < mt ? c0 + SB : mt;       // rows of its diagonal block
+        const int64_t loc = gid - start(c0);
+        // pairs in the diagonal block: sum over tj in [c0, c1) of (rb - tj)
+        const int64_t w = c1 - c0;
+        const int64_t D = w * rb - (c0 + c1 - 1) * w / 2;
+        if (loc < D) {
+            // column-major over the triangle: column c0 + u holds the rows
+            // [c0 + u, rb), M - u of them (M = rb - c0); dstart(u) = u M - u(u-1)/2
+            const int64_t M = rb - c0;
+            auto dstart = [&](int64_t u) { return u * M - u * (u - 1) / 2; };
+            const double tw2 = 2.0 * (double)M + 1.0;
+            int64_t u = (int64_t)((tw2 - sqrt(fmax(tw2 * tw2 - 8.0 * (double)loc, 0.0))) * 0.5);
+            if (u < 0) u = 0;
+            if (u > w - 1) u = w - 1;
+            while (u > 0 && dstart(u) > loc) --u;
+            while (u + 1 < w && dstart(u + 1) <= loc) ++u;
+            tj = c0 + u;
+            ti = tj + (loc - dstart(u));
+        } else {
+            // rows [rb, mt) as SB-row super-blocks, each column-major (the
+            // tiles of one column, consecutive in memory, on one XCD together)
+            const int64_t l2 = loc - D;
+            const int64_t sbi = l2 / (SB * w), inner = l2 - sbi * SB * w;
+            const int64_t rbase = rb + sbi * SB;
+            const int64_t rows = mt - rbase < SB ? mt - rbase : SB;
+            tj = c0 + inner / rows;
+            ti = rbase + inner % rows;
+        }
     }
     const int64_t trow = (ti * TILE) / tile_rows, tcol = (tj * TILE) / tile_rows;
 
@@ -780,11 +816,11 @@ static bool syrk_geometry(int64_t m, int tile, int64_t tcol0, int64_t tcol1, syr
     if (trow0 > 0) {
         if (trow0 >= g.mt) return false;
         g.npair = (g.mt - trow0) * (tcol1 - tcol0);
-        g.shape = syrk_shape{g.mt, tcol0, trow0, tcol1 - tcol0};
+        g.shape = syrk_shape{g.mt, tcol0, tcol1, trow0, tcol1 - tcol0};
     } else {
         auto start = [&](int64_t c) { return c * g.mt - c * (c - 1) / 2; };
         g.npair = start(tcol1) - start(tcol0);
-        g.shape = syrk_shape{g.mt, tcol0, 0, 0};
+        g.shape = syrk_shape{g.mt, tcol0, tcol1, 0, 0};
     }
     g.per_xcd = (g.npair + 7) / 8;
     g.tcol0 = tcol0;
