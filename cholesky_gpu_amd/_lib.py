@@ -24,6 +24,7 @@ PREC_SWITCH = -1
 PREC_NAMES = {0: "fp32", 1: "bf16", 2: "fp16", 3: "e5m2", 4: "e4m3", 5: "mxe4m3", 6: "mxe5m2"}
 
 NOALLOC_MAX_TILES = 1024        # MPC_NOALLOC_MAX_TILES of include/mpchol.h
+BLOCK_PCG_MAX = 16              # MPC_BLOCK_PCG_MAX
 DIST_RANDOM_LOG_UNIFORM, DIST_CLUSTERED, DIST_ARITHMETIC, DIST_GEOMETRIC, DIST_LOGSPACED = range(5)
 MAT_HOUSEHOLDER_SPD = 0
 
@@ -97,6 +98,10 @@ SIGNATURES = {
     "mpc_pcg_opts_default": (None, [C.POINTER(PcgOpts)]),
     "mpc_pcg": (I, [P, P, I64, P, I64, I64, I, I, P, I64, P, I64, C.POINTER(PcgOpts),
                     C.POINTER(PcgInfo)]),
+    "mpc_refine": (I, [P, P, I64, P, I64, I64, I, I, P, I64, P, I64, C.POINTER(PcgOpts),
+                       C.POINTER(PcgInfo)]),
+    "mpc_block_pcg": (I, [P, P, I64, P, I64, I64, I, I, P, I64, P, I64, C.POINTER(PcgOpts),
+                          C.POINTER(PcgInfo)]),
     "mpc_precond_cg": (I, [P, P, I64, I64, I, P, I64, P, I64, C.POINTER(FactorOpts),
                            C.POINTER(PcgOpts), C.POINTER(FactorInfo), C.POINTER(PcgInfo)]),
     "mpc_pcg64": (I, [P, P, I64, P, I64, I64, I, I, P, I64, P, I64, C.POINTER(PcgOpts),

@@ -50,6 +50,22 @@ int mpc_ws_reserve(mpc_workspace& w, size_t bytes) {
     return 0;
 }
 
+// Arrival counters for kernels whose last workgroup finishes a split
+// reduction.  Every such launch leaves its counters at zero, so they are
+// cleared only when the buffer is (re)allocated, which happens outside any
+// stream capture (before the first use at a given size).
+int* mpc_tickets(mpc_handle* h, size_t count) {
+    const size_t bytes = sizeof(int) * count;
+    if (!h->tick.ptr || bytes > h->tick.bytes) {
+        if (mpc_ws_reserve(h->tick, bytes)) return nullptr;
+        if (hipMemsetAsync(h->tick.ptr, 0, h->tick.bytes, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess) {
+            mpc_set_error("ticket buffer: clear failed");
+            return nullptr;
+        }
+    }
+    return (int*)h->tick.ptr;
+}
 
 // ---------------------------------------------------------------- small init kernels
 namespace {
@@ -174,7 +190,7 @@ extern "C" int mpc_handle_destroy(mpc_handle* h) {
     if (h->fstream) (void)hipStreamSynchronize(h->fstream);
     for (mpc_workspace* w : {&h->panel, &h->scales, &h->dinv, &h->small, &h->rowstat, &h->amat,
                              &h->pcg, &h->scratch, &h->winv, &h->gp64, &h->gp32, &h->gpvec,
-                             &h->tlist})
+                             &h->tlist, &h->tick})
         ws_free(*w);
     if (h->pcg_exec) (void)hipGraphExecDestroy(h->pcg_exec);
     if (h->ev_col) (void)hipEventDestroy(h->ev_col);
@@ -573,7 +589,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                                       sm->tile_maxrs[b], sm, o->update_prec, ladder, o->n_ladder,
                                       eps, sm->tile_prec[q], map ? map + (j + 1) + j * nt : nullptr,
                                       payload + q * pstride, (int64_t)nb * 4, scales + q * sstride,
-                                      &sm->nan_flag, 0, -1, b);
+                                      &sm->nan_flag, 0, -1, b, tr / nb, nt - (j + 1));
     };
     // trailing update of panel j on stream us restricted to SYRK tile columns
     // [t0, t1) of A22 (tiles of MPC_SYRK_TILE, or of 64 * mixed when mixed)
@@ -740,7 +756,8 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                          st, A21, s, nb, lda, tr, nullptr, sm->tile_amax[b], sm->tile_maxrs[b], sm,
                          o->update_prec, ladder, o->n_ladder, eps, sm->tile_prec[b],
                          map ? map + (j + 1) + j * nt : nullptr, payload + b * pstride,
-                         (int64_t)nb * 4, scales + b * sstride, &sm->nan_flag, 0, -1, b)))
+                         (int64_t)nb * 4, scales + b * sstride, &sm->nan_flag, 0, -1, b, tr / nb,
+                         nt - (j + 1))))
                     return rc;
                 MPC_HIP(hipEventRecord(ev[3], st));
                 if ((rc = update(j, 0, -1, st, 0))) return rc;
@@ -809,7 +826,7 @@ static void flops_from_map(const uint8_t* map, int64_t n, int nb, mpc_factor_inf
         const int64_t ntl = (s + tr - 1) / tr;
         double S[8] = {}, Q[8] = {}, D[8] = {};
         for (int64_t t = 0; t < ntl; ++t) {
-            const int p = map[(j + 1 + t) + j * nt];
+            const int p = map[(j + 1 + t * (tr / nb)) + j * nt];
             if (p >= 8) continue;
             fi->tiles[p]++;
             const double r = (double)std::min<int64_t>(tr, s - t * tr);
@@ -1306,6 +1323,259 @@ extern "C" int mpc_pcg64(mpc_handle* h, const double* dA, int64_t lda, const flo
     pcg_operator op;
     op.a64 = dA;
     return pcg_impl(h, op, lda, dL, ldl, n, nb, nrhs, dB, ldb, dX, ldx, o, info);
+}
+
+// ---------------------------------------------------------------- iterative refinement and block PCG
+namespace {
+// workspace of the two solvers below: n x nrhs blocks R, Z, P, Q, X, W and
+// the scratch of the products, inside the PCG workspace
+struct ext_bufs {
+    pcg_bufs b;
+    double *G, *Gn, *PQ, *coef, *gpart;
+    int* flag;
+    int* gtick;
+};
+
+int ext_setup(mpc_handle* h, int64_t n, int nrhs, ext_bufs& e) {
+    const size_t vec = (size_t)n * nrhs;
+    const int Bo = mpc_block_bo(n);
+    const size_t part = std::max<size_t>(
+        std::max<size_t>((size_t)mpc_symv_partial_doubles(n, nrhs),
+                         (size_t)mpc_block_prec_partial_doubles(n, Bo, nrhs)),
+        std::max<size_t>(2 * (size_t)nrhs * ((n + 2047) / 2048) + 64,
+                         (size_t)mpc_gram_partial_doubles(n, nrhs)));
+    const size_t s2 = (size_t)nrhs * nrhs;
+    const size_t nbytes = sizeof(double) * (6 * vec + part + 4 * s2 + 8 * (size_t)nrhs + 64) + 256;
+    int rc;
+    if ((rc = mpc_ws_reserve(h->pcg, nbytes))) return rc;
+    double* base = (double*)h->pcg.ptr;
+    pcg_bufs& b = e.b;
+    b = pcg_bufs{};
+    b.R = base; b.Z = b.R + vec; b.P = b.Z + vec; b.Q = b.P + vec; b.X = b.Q + vec; b.W = b.X + vec;
+    b.Y = b.W;                                   // the sweeps' work block (W is free then)
+    b.part = b.W + vec;
+    e.gpart = b.part;
+    e.G = b.part + part; e.Gn = e.G + s2; e.PQ = e.Gn + s2; e.coef = e.PQ + s2;
+    b.rr = e.coef + s2; b.bb = b.rr + nrhs; b.rz = b.bb + nrhs; b.rz_new = b.rz + nrhs;
+    b.pq = b.rz_new + nrhs;
+    e.flag = (int*)(b.pq + nrhs + 8);
+    e.gtick = mpc_tickets(h, 1);
+    if (!e.gtick) return MPC_ERR_NOMEM;
+    return 0;
+}
+
+// host copy of an nrhs-vector of sums of squares -> relative norms
+int rel_norms(hipStream_t st, const double* d, int nrhs, const std::vector<double>& nb,
+              std::vector<double>& out) {
+    std::vector<double> h(nrhs);
+    MPC_HIP(hipMemcpyAsync(h.data(), d, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    MPC_HIP(hipStreamSynchronize(st));
+    out.resize(nrhs);
+    for (int j = 0; j < nrhs; ++j) out[j] = std::sqrt(h[j]) / nb[j];
+    return 0;
+}
+
+int ext_check(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_t ldl,
+              int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
+              int64_t ldx, const mpc_pcg_opts* o, const char* who) {
+    MPC_CHECK(h && dA && dL && dB && dX, "%s: NULL argument", who);
+    MPC_CHECK(n >= 1 && lda >= n && ldl >= n && ldb >= n && ldx >= n, "%s: bad leading dimension",
+              who);
+    MPC_CHECK(nb == 64 || nb == 128 || nb == 256, "%s: nb must be 64, 128 or 256", who);
+    MPC_CHECK(o->tol > 0 && o->maxit >= 0, "%s: tol > 0 and maxit >= 0 required", who);
+    MPC_CHECK(o->x0_mode >= 0 && o->x0_mode <= 2, "%s: x0_mode must be 0, 1 or 2", who);
+    (void)nrhs;
+    return 0;
+}
+
+// x0 by mode into b.X (ld n): 0 M^-1 B, 1 zero, 2 from dX
+int ext_x0(mpc_handle* h, const float* dL, int64_t ldl, int64_t n, int nb, int nrhs,
+           const double* dB, int64_t ldb, const double* dX, int64_t ldx, int mode, pcg_bufs& b) {
+    hipStream_t st = h->stream;
+    int rc;
+    if (mode == 0) {
+        if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.R, n))) return rc;
+        return apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.X, b);
+    }
+    if (mode == 1) return mpc_launch_axpby(st, n, nrhs, 0.0, b.X, n, 0.0, b.X, n);
+    return mpc_launch_copy2d(st, n, nrhs, dX, ldx, b.X, n);
+}
+
+// R = B - A X and rr = column sums of squares of R
+int ext_residual(hipStream_t st, const pcg_operator& op, int64_t lda, int64_t n, int nrhs,
+                 const double* dB, int64_t ldb, pcg_bufs& b) {
+    int rc;
+    if ((rc = op_apply(st, op, lda, n, nrhs, b.X, n, b.Q, n, b.part))) return rc;
+    if ((rc = mpc_launch_copy2d(st, n, nrhs, dB, ldb, b.R, n))) return rc;
+    if ((rc = mpc_launch_axpby(st, n, nrhs, -1.0, b.Q, n, 1.0, b.R, n))) return rc;
+    return mpc_launch_dot(st, n, nrhs, b.R, n, b.R, n, b.rr, b.part);
+}
+
+int ext_norms_b(hipStream_t st, int64_t n, int nrhs, const double* dB, int64_t ldb, pcg_bufs& b,
+                std::vector<double>& nb) {
+    int rc;
+    if ((rc = mpc_launch_dot(st, n, nrhs, dB, ldb, dB, ldb, b.bb, b.part))) return rc;
+    std::vector<double> one(nrhs, 1.0);
+    if ((rc = rel_norms(st, b.bb, nrhs, one, nb))) return rc;
+    for (double& v : nb) if (v == 0.0) v = 1.0;
+    return 0;
+}
+}  // namespace
+
+extern "C" int mpc_refine(mpc_handle* h, const float* dA, int64_t lda, const float* dL,
+                          int64_t ldl, int64_t n, int nb, int nrhs, const double* dB, int64_t ldb,
+                          double* dX, int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info) {
+    mpc_pcg_opts def;
+    if (!o) { mpc_pcg_opts_default(&def); o = &def; }
+    int rc;
+    if ((rc = ext_check(h, dA, lda, dL, ldl, n, nb, nrhs, dB, ldb, dX, ldx, o, "refine"))) return rc;
+    MPC_CHECK(nrhs >= 1 && nrhs <= 4096, "refine: nrhs must be 1..4096");
+    MPC_HIP(hipSetDevice(h->device));
+    hipStream_t st = h->stream;
+    if ((rc = prepare_prec(h, dL, ldl, n))) return rc;
+    ext_bufs e;
+    if ((rc = ext_setup(h, n, nrhs, e))) return rc;
+    pcg_bufs& b = e.b;
+    pcg_operator op;
+    op.a32 = dA;
+    owned_events<2> tev;
+    if ((rc = tev.create())) return rc;
+    MPC_HIP(hipEventRecord(tev.e[0], st));
+    std::vector<double> nbv, rel;
+    if ((rc = ext_norms_b(st, n, nrhs, dB, ldb, b, nbv))) return rc;
+    if ((rc = ext_x0(h, dL, ldl, n, nb, nrhs, dB, ldb, dX, ldx, o->x0_mode, b))) return rc;
+    int k = 0;
+    bool conv = false;
+    for (;;) {
+        if ((rc = ext_residual(st, op, lda, n, nrhs, dB, ldb, b))) return rc;
+        if ((rc = rel_norms(st, b.rr, nrhs, nbv, rel))) return rc;
+        const double res = *std::max_element(rel.begin(), rel.end());
+        if (res <= o->tol) { conv = true; break; }
+        if (k == o->maxit || !(res < INFINITY)) break;
+        // x += M^-1 (b - A x)
+        if ((rc = apply_prec(h, dL, ldl, n, nb, nrhs, b.R, b.Z, b))) return rc;
+        if ((rc = mpc_launch_axpby(st, n, nrhs, 1.0, b.Z, n, 1.0, b.X, n))) return rc;
+        ++k;
+    }
+    if ((rc = mpc_launch_copy2d(st, n, nrhs, b.X, n, dX, ldx))) return rc;
+    MPC_HIP(hipEventRecord(tev.e[1], st));
+    MPC_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, tev.e[0], tev.e[1]);
+    mpc_pcg_info pi;
+    std::memset(&pi, 0, sizeof pi);
+    pi.iters = k;
+    pi.relres = *std::max_element(rel.begin(), rel.end());
+    for (double r : rel) pi.converged += r <= o->tol;
+    pi.kappa_est = 0.0;
+    pi.ms_total = ms;
+    if (info) *info = pi;
+    return conv ? 0 : MPC_ERR_NOT_CONVERGED;
+}
+
+extern "C" int mpc_block_pcg(mpc_handle* h, const float* dA, int64_t lda, const float* dL,
+                             int64_t ldl, int64_t n, int nb, int nrhs, const double* dB,
+                             int64_t ldb, double* dX, int64_t ldx, const mpc_pcg_opts* o,
+                             mpc_pcg_info* info) {
+    mpc_pcg_opts def;
+    if (!o) { mpc_pcg_opts_default(&def); o = &def; }
+    int rc;
+    if ((rc = ext_check(h, dA, lda, dL, ldl, n, nb, nrhs, dB, ldb, dX, ldx, o, "block_pcg")))
+        return rc;
+    MPC_CHECK(nrhs >= 1 && nrhs <= MPC_BLOCK_PCG_MAX, "block_pcg: nrhs must be 1..%d",
+              MPC_BLOCK_PCG_MAX);
+    MPC_HIP(hipSetDevice(h->device));
+    hipStream_t st = h->stream;
+    if ((rc = prepare_prec(h, dL, ldl, n))) return rc;
+    ext_bufs e;
+    if ((rc = ext_setup(h, n, nrhs, e))) return rc;
+    pcg_bufs& b = e.b;
+    const int s = nrhs;
+    pcg_operator op;
+    op.a32 = dA;
+    owned_events<2> tev;
+    if ((rc = tev.create())) return rc;
+    MPC_HIP(hipEventRecord(tev.e[0], st));
+    std::vector<double> nbv, rel;
+    if ((rc = ext_norms_b(st, n, s, dB, ldb, b, nbv))) return rc;
+    if ((rc = ext_x0(h, dL, ldl, n, nb, s, dB, ldb, dX, ldx, o->x0_mode, b))) return rc;
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(256), 0, st, e.flag, 1, 0);
+    MPC_LAUNCH_CHECK();
+    int k = 0, restarts = 0, hflag = 0;
+    bool restart = true, conv = false;
+    // R = B - A X, Z = M^-1 R, P = Z, G = Z^T R
+    auto start = [&]() -> int {
+        int r;
+        if ((r = ext_residual(st, op, lda, n, s, dB, ldb, b))) return r;
+        if ((r = apply_prec(h, dL, ldl, n, nb, s, b.R, b.Z, b))) return r;
+        if ((r = mpc_launch_copy2d(st, n, s, b.Z, n, b.P, n))) return r;
+        return mpc_launch_gram(st, n, s, b.Z, n, b.R, n, e.gpart, e.G, e.gtick);
+    };
+    for (;;) {
+        if (restart) {
+            if ((rc = start())) return rc;
+            restart = false;
+        } else if ((rc = mpc_launch_dot(st, n, s, b.R, n, b.R, n, b.rr, b.part))) {
+            return rc;
+        }
+        MPC_HIP(hipMemcpyAsync(&hflag, e.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+        if ((rc = rel_norms(st, b.rr, s, nbv, rel))) return rc;
+        if (hflag) break;                        // the block recurrence broke down
+        const double res = *std::max_element(rel.begin(), rel.end());
+        if (res <= o->tol) {
+            // confirm on the true residual; a lagging one restarts from X
+            if ((rc = ext_residual(st, op, lda, n, s, dB, ldb, b))) return rc;
+            if ((rc = rel_norms(st, b.rr, s, nbv, rel))) return rc;
+            if (*std::max_element(rel.begin(), rel.end()) <= o->tol) { conv = true; break; }
+            if (++restarts > 8) break;
+            restart = true;
+            continue;
+        }
+        if (k == o->maxit || !(res < INFINITY)) break;
+        ++k;
+        // Q = A P; alpha = (P^T Q)^-1 G; X += P alpha; R -= Q alpha
+        if ((rc = op_apply(st, op, lda, n, s, b.P, n, b.Q, n, b.part))) return rc;
+        if ((rc = mpc_launch_gram(st, n, s, b.P, n, b.Q, n, e.gpart, e.PQ, e.gtick))) return rc;
+        if ((rc = mpc_launch_spd_solve(st, s, e.PQ, e.G, e.coef, e.flag))) return rc;
+        if ((rc = mpc_launch_block_axpy(st, n, s, b.P, n, e.coef, 1.0, nullptr, 0, b.X, n))) return rc;
+        if ((rc = mpc_launch_block_axpy(st, n, s, b.Q, n, e.coef, -1.0, nullptr, 0, b.R, n))) return rc;
+        // Z = M^-1 R; beta = G^-1 Z^T R; P = Z + P beta
+        if ((rc = apply_prec(h, dL, ldl, n, nb, s, b.R, b.Z, b))) return rc;
+        if ((rc = mpc_launch_gram(st, n, s, b.Z, n, b.R, n, e.gpart, e.Gn, e.gtick))) return rc;
+        if ((rc = mpc_launch_spd_solve(st, s, e.G, e.Gn, e.coef, e.flag))) return rc;
+        if ((rc = mpc_launch_block_axpy(st, n, s, b.P, n, e.coef, 1.0, b.Z, n, b.P, n))) return rc;
+        std::swap(e.G, e.Gn);
+    }
+    if ((rc = mpc_launch_copy2d(st, n, s, b.X, n, dX, ldx))) return rc;
+    MPC_HIP(hipEventRecord(tev.e[1], st));
+    MPC_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, tev.e[0], tev.e[1]);
+    mpc_pcg_info pi;
+    std::memset(&pi, 0, sizeof pi);
+    pi.iters = k;
+    pi.kappa_est = 0.0;
+    if (hflag && o->maxit > k) {
+        // breakdown (P^T A P or Z^T R singular in working precision): the
+        // columns finish as independent PCGs from the current iterate
+        mpc_pcg_opts o2 = *o;
+        o2.x0_mode = 2;
+        o2.maxit = o->maxit - k;
+        mpc_pcg_info p2;
+        const int r2 = pcg_impl(h, op, lda, dL, ldl, n, nb, s, dB, ldb, dX, ldx, &o2, &p2);
+        if (r2 && r2 != MPC_ERR_NOT_CONVERGED) return r2;
+        pi.iters = k + p2.iters;
+        pi.converged = p2.converged;
+        pi.relres = p2.relres;
+        pi.ms_total = ms + p2.ms_total;
+        if (info) *info = pi;
+        return r2;
+    }
+    pi.relres = *std::max_element(rel.begin(), rel.end());
+    for (double r : rel) pi.converged += r <= o->tol;
+    pi.ms_total = ms;
+    if (info) *info = pi;
+    return conv ? 0 : MPC_ERR_NOT_CONVERGED;
 }
 
 extern "C" int mpc_precond_cg(mpc_handle* h, const float* dA, int64_t lda, int64_t n, int nrhs,

@@ -211,7 +211,7 @@ __global__ __launch_bounds__(256) void quantize_kernel(
     const uint32_t* __restrict__ tile_maxrs, const mpc_small* __restrict__ smc, rule_args ra,
     uint8_t* __restrict__ prec_out, uint8_t* __restrict__ map_col, uint8_t* __restrict__ payload,
     int64_t row_stride, uint8_t* __restrict__ scales, int* __restrict__ nan_flag, int64_t row0,
-    int mu_slot) {
+    int mu_slot, int map_rep, int64_t map_len) {
     if (smc && smc->info != 0) return;
     const int64_t r0 = row0 + (int64_t)blockIdx.x * 32;
     const int r = threadIdx.x & 31, qq0 = threadIdx.x >> 5;
@@ -225,7 +225,9 @@ __global__ __launch_bounds__(256) void quantize_kernel(
         prec = decide(ra, __uint_as_float(tile_maxrs[tile]), mu, __uint_as_float(tile_amax[tile]));
         if (r0 % tile_rows == 0 && threadIdx.x == 0) {
             if (prec_out) prec_out[tile] = (uint8_t)prec;
-            if (map_col) map_col[tile] = (uint8_t)prec;
+            // the map is in nb blocks: a tile of map_rep blocks fills each of them
+            for (int u = 0; u < map_rep && map_col; ++u)
+                if ((int64_t)tile * map_rep + u < map_len) map_col[tile * map_rep + u] = (uint8_t)prec;
         }
     }
     const int kb = k >> 5;
@@ -352,7 +354,9 @@ int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int
                            const uint32_t* tile_maxrs, mpc_small* sm, int update_prec,
                            const int* ladder, int n_ladder, double eps_prime, uint8_t* prec_out,
                            uint8_t* map_col, uint8_t* payload, int64_t row_stride, uint8_t* scales,
-                           int* nan_flag, int64_t row0, int64_t row1, int mu_slot) {
+                           int* nan_flag, int64_t row0, int64_t row1, int mu_slot, int map_rep,
+                           int64_t map_len) {
+    if (map_len < 0) map_len = ((m + tile_rows - 1) / tile_rows) * (int64_t)map_rep;
     if (row1 < 0 || row1 > m) row1 = m;
     if (row1 <= row0) return 0;
     if (k % 32 || tile_rows % 32 || row0 % 32 || (row1 != m && row1 % 32)) {
@@ -367,7 +371,8 @@ int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int
     ra.eps_prime = eps_prime;
     hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)((row1 - row0 + 31) / 32)), dim3(256), 0, st,
                        P, m, k, ld, tile_rows, prec_in, tile_amax, tile_maxrs, sm, ra, prec_out,
-                       map_col, payload, row_stride, scales, nan_flag, row0, mu_slot & 1);
+                       map_col, payload, row_stride, scales, nan_flag, row0, mu_slot & 1, map_rep,
+                       map_len);
     MPC_LAUNCH_CHECK();
     return 0;
 }

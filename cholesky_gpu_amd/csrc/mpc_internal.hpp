@@ -7,6 +7,29 @@
 #include <vector>
 #include "../../include/mpchol.h"
 
+// ---------------------------------------------------------------- device helpers
+// True in every thread of the one workgroup that arrives last (of `total`)
+// at counter *t, which that workgroup resets to zero for the next launch.
+// The release fence before the increment and the acquire fence after it make
+// every workgroup's earlier global writes visible to the last one.  Nothing
+// waits: the other workgroups simply end.  The agent-scope release writes
+// back the XCD's L2 (about 1 us each, serialised per XCD on gfx950), so this
+// is for launches of a few workgroups only.
+static __device__ __forceinline__ bool mpc_last_arrival(int* t, int total) {
+    __shared__ int flag;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int last = atomicAdd(t, 1) == total - 1;
+        if (last) atomicExch(t, 0);
+        flag = last;
+    }
+    __syncthreads();
+    const bool last = flag != 0;
+    if (last) __threadfence();
+    return last;
+}
+
 // ---------------------------------------------------------------- errors
 void mpc_set_error(const char* fmt, ...);
 
@@ -101,6 +124,7 @@ struct mpc_handle {
     mpc_workspace gp64;       // GP: FP64 operator A (n x n)
     mpc_workspace gp32;       // GP: FP32 copy of A (factor source)
     mpc_workspace gpvec;      // GP: right-hand sides, solutions, U, V, partials
+    mpc_workspace tick;       // arrival counters of split reductions (zero between launches)
     // provenance of the dinv blocks: the factor they belong to
     const float* dinv_owner = nullptr;
     int64_t      dinv_n = 0;
@@ -123,6 +147,9 @@ __device__ __forceinline__ void mpc_lds_barrier() {
 
 // grow-only device allocation in a handle workspace
 int mpc_ws_reserve(mpc_workspace& w, size_t bytes);
+// count zeroed arrival counters (grow-only, cleared when allocated); nullptr
+// with the error set on failure
+int* mpc_tickets(mpc_handle* h, size_t count);
 
 // noalloc.hip: looks up the current device's static scratch array (once per
 // device; mpc_handle_create calls it)
@@ -169,7 +196,8 @@ int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int
                            const uint32_t* tile_maxrs, mpc_small* sm, int update_prec,
                            const int* ladder, int n_ladder, double eps_prime, uint8_t* prec_out,
                            uint8_t* map_col, uint8_t* payload, int64_t row_stride, uint8_t* scales,
-                           int* nan_flag, int64_t row0 = 0, int64_t row1 = -1, int mu_slot = 0);
+                           int* nan_flag, int64_t row0 = 0, int64_t row1 = -1, int mu_slot = 0,
+                           int map_rep = 1, int64_t map_len = -1);
 int mpc_launch_tile_amax(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
                          int tile_rows, uint32_t* tile_amax);
 // k_syrk.hip
@@ -271,6 +299,18 @@ int mpc_launch_probes(hipStream_t st, double* Z, int64_t ldz, int64_t n, int npr
                       int unit);
 int mpc_launch_convert_f8(hipStream_t st, const float* x, uint8_t* y, int64_t count, int prec);
 int mpc_launch_dinv_from_L(hipStream_t st, const float* L, int64_t ldl, int64_t n, float* dinv);
+// block PCG helpers (k_pcg.hip): s <= MPC_BLOCK_PCG_MAX columns
+int64_t mpc_gram_partial_doubles(int64_t n, int s);
+// G = X^T Y (s x s); tick: one zeroed counter (left zeroed)
+int mpc_launch_gram(hipStream_t st, int64_t n, int s, const double* X, int64_t ldx,
+                    const double* Y, int64_t ldy, double* part, double* G, int* tick);
+// X = G^-1 H (s x s, G SPD); *flag = 1 and X = 0 on a non-positive pivot
+int mpc_launch_spd_solve(hipStream_t st, int s, const double* G, const double* H, double* X,
+                         int* flag);
+// Y = (Z ? Z : Y) + a M C (n x s blocks, s x s C); Y may be M
+int mpc_launch_block_axpy(hipStream_t st, int64_t n, int s, const double* M, int64_t ldm,
+                          const double* Cm, double a, const double* Z, int64_t ldz, double* Y,
+                          int64_t ldy);
 // k_tri.hip
 int mpc_build_block_inverses_range(hipStream_t st, const float* L, int64_t ldl, int64_t n, int Bo,
                                    const float* dinv, float* W, float* WT, int64_t I0, int64_t I1);

@@ -174,6 +174,38 @@ def pcg(h, A, Lf, B, nb=256, opts=None, X=None):
     return X, pi, rc
 
 
+def refine(h, A, Lf, B, nb=256, opts=None, X=None):
+    """Iterative refinement X += M^-1 (B - A X) with the factor Lf (mpc_refine).
+    B: (nrhs, n) float64.  Returns (X, info, rc); info.iters = corrections."""
+    n = A.shape[0]
+    B = B.reshape(-1, n).contiguous()
+    nrhs = B.shape[0]
+    X = X if X is not None else torch.zeros_like(B)
+    pi = L.PcgInfo()
+    opts = opts or pcg_opts(maxit=50)
+    _pre(h)
+    rc = h.lib.mpc_refine(h.h, _ptr(_dev(A)), n, _ptr(_dev(Lf)), n, n, nb, nrhs, _ptr(_dev(B)), n,
+                          _ptr(_dev(X)), n, C.byref(opts), C.byref(pi))
+    check(rc, "mpc_refine", allow=(L.ERR_NOT_CONVERGED,))
+    return X, pi, rc
+
+
+def block_pcg(h, A, Lf, B, nb=256, opts=None, X=None):
+    """Block PCG over the rows of B (nrhs <= 16, (nrhs, n) float64) together
+    (mpc_block_pcg).  Returns (X, info, rc)."""
+    n = A.shape[0]
+    B = B.reshape(-1, n).contiguous()
+    nrhs = B.shape[0]
+    X = X if X is not None else torch.zeros_like(B)
+    pi = L.PcgInfo()
+    opts = opts or pcg_opts()
+    _pre(h)
+    rc = h.lib.mpc_block_pcg(h.h, _ptr(_dev(A)), n, _ptr(_dev(Lf)), n, n, nb, nrhs,
+                             _ptr(_dev(B)), n, _ptr(_dev(X)), n, C.byref(opts), C.byref(pi))
+    check(rc, "mpc_block_pcg", allow=(L.ERR_NOT_CONVERGED,))
+    return X, pi, rc
+
+
 def pcg_last_coeffs(h, maxn=4096):
     """(alphas, betas) of the first right-hand side of the handle's last PCG
     solve, as numpy float64 arrays (the CG coefficients behind kappa_est)."""

@@ -181,6 +181,34 @@ int mpc_pcg(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_
             int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
             int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info);
 
+/* ---- iterative refinement and block PCG (multi-RHS) --------------------- */
+/* Mixed-precision iterative refinement around the factor: x0 (x0_mode 0:
+   M^-1 b, 1: 0, 2: from dX), then x += M^-1 (b - A x) until every column's
+   FP64 true residual ||b - A x|| / ||b|| <= tol.  The residual and the
+   update are FP64, the correction goes through the FP32 factor L.  The
+   reference has no such loop; its error trace of one
+   (src/e5m2_error_f_cond.csv:1-5, error per refinement step) is what it
+   produces.  Converges when the spectral radius of I - M^-1 A is below 1
+   (for FP8-update factors of ill-conditioned A it may not be: PCG then
+   still converges).  info->iters = corrections applied.  Arguments and
+   returns as mpc_pcg (stop_rule, graph and check_every are not used). */
+int mpc_refine(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_t ldl,
+               int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
+               int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info);
+
+/* Block PCG (O'Leary 1980) over the nrhs columns together: one block
+   Krylov space, s x s = nrhs x nrhs FP64 coefficient solves per step
+   instead of the scalar alpha, beta of the single-vector loop
+   (mixed_cholesky.cu:180-217).  nrhs <= MPC_BLOCK_PCG_MAX.  If the block
+   recurrence breaks down (P^T A P or Z^T R not positive definite in FP64)
+   the columns finish as independent PCGs (mpc_pcg) from the current
+   iterate.  info->kappa_est is 0 (no Lanczos estimate).  Arguments and
+   returns as mpc_pcg (stop_rule, graph and check_every are not used). */
+#define MPC_BLOCK_PCG_MAX 16
+int mpc_block_pcg(mpc_handle* h, const float* dA, int64_t lda, const float* dL, int64_t ldl,
+                  int64_t n, int nb, int nrhs, const double* dB, int64_t ldb, double* dX,
+                  int64_t ldx, const mpc_pcg_opts* o, mpc_pcg_info* info);
+
 /* ---- factor + solve (precond_CG, mixed_cholesky.cu:76-229) ------------- */
 /* Factors a workspace copy of A (A itself is not modified), then runs PCG.
    Returns 0 (converged), MPC_ERR_NOT_CONVERGED, or factor info > 0. */

@@ -751,6 +751,7 @@ int orc_factor_mixed(float* A, int64_t n, int64_t lda, const orc_factor_opts* o,
     const int64_t nb = o->nb;
     const int64_t nt = (n + nb - 1) / nb;
     const int tr = o->tile_rows > 0 ? o->tile_rows : (int)nb;
+    if (tr < nb || tr % nb) return -3;      /* tiles are whole nb blocks */
     orc_factor_info inf;
     memset(&inf, 0, sizeof inf);
     double mu = INFINITY;
@@ -804,7 +805,7 @@ int orc_factor_mixed(float* A, int64_t n, int64_t lda, const orc_factor_opts* o,
         for (int64_t t = 0; t < ntl; ++t) {
             int p;
             if (map_in) {
-                p = map_in[(j + 1 + t) + j * nt];
+                p = map_in[(j + 1 + t * (tr / nb)) + j * nt];
             } else if (o->update_prec >= 0) {
                 p = o->update_prec;
             } else {
@@ -819,7 +820,9 @@ int orc_factor_mixed(float* A, int64_t n, int64_t lda, const orc_factor_opts* o,
                 p = orc_rule_minpivot_amax(o->ladder, o->n_ladder, o->eps_prime, mx, mu, am);
             }
             ptile[t] = (uint8_t)p;
-            if (map_out) map_out[(j + 1 + t) + j * nt] = (uint8_t)p;
+            /* the map is in nb blocks: a tile of tr / nb blocks fills each */
+            for (int64_t u = 0; map_out && u < tr / nb; ++u)
+                if (j + 1 + t * (tr / nb) + u < nt) map_out[(j + 1 + t * (tr / nb) + u) + j * nt] = (uint8_t)p;
         }
         if (map_out) map_out[j + j * nt] = ORC_FP32;
         /* quantise the panel (bs must be a multiple of 32 when s > 0) */
@@ -954,6 +957,209 @@ int orc_pcg(const float* A, int64_t lda, const float* L, int64_t ldl, int64_t n,
     if (relres) *relres = sqrt(dotd(q, q, n)) / nb;
     free(r); free(z); free(p); free(q);
     return done ? k : -1;
+}
+
+/* Mixed-precision iterative refinement around the low-precision factor (no
+   reference routine: the reference's error trace of such a scheme is
+   src/e5m2_error_f_cond.csv:1-5, one row per refinement step).  FP64 residual
+   and update, correction solve through L:
+     x0 = M^-1 b (x0_mode 0) or 0 (x0_mode 1)
+     repeat: r = b - A x;  stop if ||r|| / ||b|| <= tol;  x += M^-1 r
+   Returns the number of corrections applied if converged, -1 at maxit.
+   hist (optional, maxit + 1) receives ||r_k|| / ||b|| for k = 0 .. steps. */
+int orc_refine(const float* A, int64_t lda, const float* L, int64_t ldl, int64_t n,
+               const double* b, double* x, const orc_pcg_opts* o, double* relres,
+               double* hist) {
+    double* r = (double*)malloc(sizeof(double) * n);
+    double nb = sqrt(dotd(b, b, n));
+    if (nb == 0.0) nb = 1.0;
+    if (o->x0_mode == 0) {
+        memcpy(x, b, sizeof(double) * n);
+        trsv_f32L_f64(L, n, ldl, x, 0);
+        trsv_f32L_f64(L, n, ldl, x, 1);
+    } else {
+        memset(x, 0, sizeof(double) * n);
+    }
+    int k = 0, ret = -1;
+    for (;;) {
+        orc_symv_f32_f64(A, n, lda, x, r);
+        for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+        const double res = sqrt(dotd(r, r, n)) / nb;
+        if (hist) hist[k] = res;
+        if (relres) *relres = res;
+        if (res <= o->tol) { ret = k; break; }
+        if (k == o->maxit) break;
+        trsv_f32L_f64(L, n, ldl, r, 0);
+        trsv_f32L_f64(L, n, ldl, r, 1);
+        for (int64_t i = 0; i < n; ++i) x[i] += r[i];
+        ++k;
+    }
+    free(r);
+    return ret;
+}
+
+/* s x s helpers of the block PCG (column-major, FP64).  spd_solve:
+   X = G^-1 H through the Cholesky factor of G's lower triangle; returns 1
+   (and X = 0) if a pivot is not above 1e-13 times its diagonal entry: G is
+   singular to working precision and the block recurrence has broken down. */
+static int spd_solve(const double* G, const double* H, double* X, int s) {
+    double* C = (double*)malloc(sizeof(double) * s * s);
+    int bad = 0;
+    for (int j = 0; j < s; ++j) {
+        const double g = G[j + j * s];
+        double d = g;
+        for (int k = 0; k < j; ++k) d -= C[j + k * s] * C[j + k * s];
+        if (!(d > 1e-13 * g) || !(d < INFINITY)) { bad = 1; break; }
+        d = sqrt(d);
+        C[j + j * s] = d;
+        for (int i = j + 1; i < s; ++i) {
+            double v = G[i + j * s];
+            for (int k = 0; k < j; ++k) v -= C[i + k * s] * C[j + k * s];
+            C[i + j * s] = v / d;
+        }
+    }
+    for (int c = 0; c < s; ++c) {
+        double* x = X + c * s;
+        if (bad) { for (int i = 0; i < s; ++i) x[i] = 0.0; continue; }
+        for (int i = 0; i < s; ++i) {               /* C y = h */
+            double v = H[i + c * s];
+            for (int k = 0; k < i; ++k) v -= C[i + k * s] * x[k];
+            x[i] = v / C[i + i * s];
+        }
+        for (int i = s - 1; i >= 0; --i) {          /* C^T x = y */
+            double v = x[i];
+            for (int k = i + 1; k < s; ++k) v -= C[k + i * s] * x[k];
+            x[i] = v / C[i + i * s];
+        }
+    }
+    free(C);
+    return bad;
+}
+
+/* G = X^T Y for n x s blocks (leading dimension n) */
+static void gram(const double* X, const double* Y, int64_t n, int s, double* G) {
+    for (int b = 0; b < s; ++b)
+        for (int a = 0; a < s; ++a) G[a + b * s] = dotd(X + a * n, Y + b * n, n);
+}
+
+/* Y (+)= M C for n x s blocks M, Y and an s x s C; Z (optional) replaces Y's
+   old value: Y = Z + M C.  In place on Y == M (each row read first). */
+static void block_axpy(const double* M, const double* Cm, double a, const double* Z, double* Y,
+                       int64_t n, int s) {
+    double* row = (double*)malloc(sizeof(double) * s);
+    for (int64_t r = 0; r < n; ++r) {
+        for (int k = 0; k < s; ++k) row[k] = M[r + k * n];
+        for (int j = 0; j < s; ++j) {
+            double v = 0.0;
+            for (int k = 0; k < s; ++k) v += row[k] * Cm[k + j * s];
+            const double base = Z ? Z[r + j * n] : Y[r + j * n];
+            Y[r + j * n] = base + a * v;
+        }
+    }
+    free(row);
+}
+
+static void prec_block(const float* L, int64_t ldl, int64_t n, int s, double* Z) {
+    for (int j = 0; j < s; ++j) {
+        trsv_f32L_f64(L, n, ldl, Z + j * n, 0);
+        trsv_f32L_f64(L, n, ldl, Z + j * n, 1);
+    }
+}
+
+/* Block PCG (O'Leary, "The block conjugate gradient algorithm and related
+   methods", Linear Algebra Appl. 29, 1980, preconditioned form) over the s
+   right-hand sides together, as the block counterpart of the single-vector
+   loop of mixed_cholesky.cu:180-217:
+     R = B - A X, Z = M^-1 R, P = Z, G = Z^T R
+     repeat: Q = A P; alpha = (P^T Q)^-1 G; X += P alpha; R -= Q alpha;
+             Z = M^-1 R; Gn = Z^T R; beta = G^-1 Gn; P = Z + P beta; G = Gn
+   Stops when every column's recursive residual ||R_j|| / ||B_j|| <= tol and
+   the true residual confirms it (a lagging true residual restarts the
+   recurrence from X).  B, X: n x s, leading dimension n.  Returns iterations
+   if converged, -1 at maxit, -2 if a Cholesky of P^T Q or G fails (the block
+   recurrence broke down; X holds the last iterate).  relres (s values)
+   receives the final true relative residuals. */
+int orc_block_pcg(const float* A, int64_t lda, const float* L, int64_t ldl, int64_t n, int s,
+                  const double* B, double* X, const orc_pcg_opts* o, double* relres) {
+    const int64_t ns = n * (int64_t)s;
+    double* R = (double*)malloc(sizeof(double) * ns);
+    double* Z = (double*)malloc(sizeof(double) * ns);
+    double* P = (double*)malloc(sizeof(double) * ns);
+    double* Q = (double*)malloc(sizeof(double) * ns);
+    double* G = (double*)malloc(sizeof(double) * s * s);
+    double* Gn = (double*)malloc(sizeof(double) * s * s);
+    double* PQ = (double*)malloc(sizeof(double) * s * s);
+    double* al = (double*)malloc(sizeof(double) * s * s);
+    double* nb = (double*)malloc(sizeof(double) * s);
+    for (int j = 0; j < s; ++j) {
+        nb[j] = sqrt(dotd(B + j * n, B + j * n, n));
+        if (nb[j] == 0.0) nb[j] = 1.0;
+    }
+    if (o->x0_mode == 0) {
+        memcpy(X, B, sizeof(double) * ns);
+        prec_block(L, ldl, n, s, X);
+    } else {
+        memset(X, 0, sizeof(double) * ns);
+    }
+    int k = 0, ret = -1, restart = 1;
+    for (;;) {
+        if (restart) {
+            for (int j = 0; j < s; ++j) orc_symv_f32_f64(A, n, lda, X + j * n, R + j * n);
+            for (int64_t i = 0; i < ns; ++i) R[i] = B[i] - R[i];
+            memcpy(Z, R, sizeof(double) * ns);
+            prec_block(L, ldl, n, s, Z);
+            memcpy(P, Z, sizeof(double) * ns);
+            gram(Z, R, n, s, G);
+            restart = 0;
+        }
+        double res = 0.0;
+        for (int j = 0; j < s; ++j) {
+            const double rj = sqrt(dotd(R + j * n, R + j * n, n)) / nb[j];
+            res = rj > res ? rj : res;
+        }
+        if (res <= o->tol) {                     /* confirm on the true residual */
+            double tr = 0.0;
+            for (int j = 0; j < s; ++j) orc_symv_f32_f64(A, n, lda, X + j * n, Q + j * n);
+            for (int j = 0; j < s; ++j) {
+                double q = 0.0;
+                for (int64_t i = 0; i < n; ++i) {
+                    const double d = B[i + j * n] - Q[i + j * n];
+                    q += d * d;
+                }
+                q = sqrt(q) / nb[j];
+                tr = q > tr ? q : tr;
+            }
+            if (tr <= o->tol) { ret = k; break; }
+            restart = 1;
+            continue;
+        }
+        if (k == o->maxit) break;
+        ++k;
+        for (int j = 0; j < s; ++j) orc_symv_f32_f64(A, n, lda, P + j * n, Q + j * n);
+        gram(P, Q, n, s, PQ);
+        if (spd_solve(PQ, G, al, s)) { ret = -2; break; }
+        block_axpy(P, al, 1.0, NULL, X, n, s);
+        block_axpy(Q, al, -1.0, NULL, R, n, s);
+        memcpy(Z, R, sizeof(double) * ns);
+        prec_block(L, ldl, n, s, Z);
+        gram(Z, R, n, s, Gn);
+        if (spd_solve(G, Gn, al, s)) { ret = -2; break; }
+        block_axpy(P, al, 1.0, Z, P, n, s);
+        memcpy(G, Gn, sizeof(double) * s * s);
+    }
+    if (relres) {
+        for (int j = 0; j < s; ++j) orc_symv_f32_f64(A, n, lda, X + j * n, Q + j * n);
+        for (int j = 0; j < s; ++j) {
+            double q = 0.0;
+            for (int64_t i = 0; i < n; ++i) {
+                const double d = B[i + j * n] - Q[i + j * n];
+                q += d * d;
+            }
+            relres[j] = sqrt(q) / nb[j];
+        }
+    }
+    free(R); free(Z); free(P); free(Q); free(G); free(Gn); free(PQ); free(al); free(nb);
+    return ret;
 }
 
 /* Sturm-sequence bisection for the extreme eigenvalues of the Lanczos

@@ -20,6 +20,11 @@
  *   orc_factor_mixed ......... FP64 oracle backward error, all-FP32 path vs
  *                              numpy, all-ones SYRK known answer
  *   orc_pcg .................. exact FP64 solve, iteration-count invariants
+ *   orc_refine ............... exact factor converges in one correction; error
+ *                              contraction bounded by ||I - M^-1 A|| (numpy);
+ *                              fixed point equals the FP64 solve
+ *   orc_block_pcg ............ s = 1 equals orc_pcg; k distinct eigenvalues
+ *                              converge in ceil(k / s) block steps; FP64 solve
  *   orc_hh_* ................. spectrum of H*Lambda*H^T equals Lambda (numpy
  *                              eigvalsh), symmetry
  *   orc_philox4x32 ........... Random123 published known-answer vectors
@@ -166,6 +171,20 @@ typedef struct {
 int orc_pcg(const float* A, int64_t lda, const float* L, int64_t ldl, int64_t n,
             const double* b, double* x, const orc_pcg_opts* o, double* relres,
             double* alphas, double* betas);
+
+/* Iterative refinement x += M^-1 (b - A x) from x0 (x0_mode 0: M^-1 b,
+   1: 0) until the FP64 true residual meets tol.  Returns corrections
+   applied, -1 at maxit; hist (optional, maxit + 1) gets the residual of
+   every iterate. */
+int orc_refine(const float* A, int64_t lda, const float* L, int64_t ldl, int64_t n,
+               const double* b, double* x, const orc_pcg_opts* o, double* relres,
+               double* hist);
+
+/* Block PCG (O'Leary 1980) over s right-hand sides (B, X: n x s, ld n).
+   Returns iterations, -1 at maxit, -2 on breakdown of the block recurrence;
+   relres (s values) gets the final true relative residuals. */
+int orc_block_pcg(const float* A, int64_t lda, const float* L, int64_t ldl, int64_t n, int s,
+                  const double* B, double* X, const orc_pcg_opts* o, double* relres);
 
 /* cond estimate of the preconditioned operator from CG coefficients
    (Lanczos tridiagonal eigenvalue extremes, bisection) */

@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "_build", "liboracle.so")
+# MPCHOL_ORACLE_LIB selects another build of oracle.c (the sanitizer build
+# of `make -C oracle sanitize`)
+_LIB = os.environ.get("MPCHOL_ORACLE_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 _REF = os.path.join(_HERE, "_ref", "libref_micro_chol.so")
 
 FP32, BF16, FP16, E5M2, E4M3, MXE4M3, MXE5M2 = range(7)
@@ -109,6 +111,8 @@ def _declare(L):
         "orc_rule_minpivot": (C.c_int, [P, C.c_int, D, D, D]),
         "orc_rule_minpivot_amax": (C.c_int, [P, C.c_int, D, D, D, D]),
         "orc_pcg": (C.c_int, [P, I64, P, I64, I64, P, P, C.POINTER(PcgOpts), P, P, P]),
+        "orc_refine": (C.c_int, [P, I64, P, I64, I64, P, P, C.POINTER(PcgOpts), P, P]),
+        "orc_block_pcg": (C.c_int, [P, I64, P, I64, I64, C.c_int, P, P, C.POINTER(PcgOpts), P]),
         "orc_lanczos_cond": (D, [P, P, C.c_int]),
         "orc_num_threads": (C.c_int, []),
     }
@@ -385,3 +389,33 @@ def pcg(A, L, b, tol=1e-10, maxit=500, stop_rule=0, x0_mode=0):
     k = it if it > 0 else (maxit if it < 0 else 0)
     kappa = lib().orc_lanczos_cond(_p(al), _p(be), k) if k > 0 else 1.0
     return x, it, rr.value, kappa
+
+
+def refine(A, L, b, tol=1e-10, maxit=50, x0_mode=0):
+    """orc_refine: x += M^-1 (b - A x) until the FP64 true residual meets
+    tol.  Returns (x, corrections or -1, final relres, residual history)."""
+    A = np.asfortranarray(A, np.float32)
+    L = np.asfortranarray(L, np.float32)
+    b = np.ascontiguousarray(b, np.float64)
+    n = b.size
+    x = np.zeros(n)
+    o = PcgOpts(tol, maxit, 0, x0_mode)
+    rr = C.c_double(0)
+    hist = np.full(maxit + 1, np.nan)
+    it = lib().orc_refine(_p(A), n, _p(L), n, n, _p(b), _p(x), C.byref(o), C.byref(rr), _p(hist))
+    k = it if it >= 0 else maxit
+    return x, it, rr.value, hist[:k + 1].copy()
+
+
+def block_pcg(A, L, B, tol=1e-10, maxit=500, x0_mode=0):
+    """orc_block_pcg over the columns of B (n x s) together.  Returns
+    (X, iterations or -1 / -2, per-column final true relres)."""
+    A = np.asfortranarray(A, np.float32)
+    L = np.asfortranarray(L, np.float32)
+    B = np.asfortranarray(B, np.float64)
+    n, s = B.shape
+    X = np.zeros((n, s), order="F")
+    o = PcgOpts(tol, maxit, 0, x0_mode)
+    rel = np.zeros(s)
+    it = lib().orc_block_pcg(_p(A), n, _p(L), n, n, s, _p(B), _p(X), C.byref(o), _p(rel))
+    return X, it, rel

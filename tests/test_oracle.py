@@ -286,6 +286,113 @@ def test_pcg_exact_and_iterations():
     assert it3 > 0
 
 
+def _diag_spd(n, kappa, seed):
+    """Diagonal SPD with n distinct eigenvalues (exact in FP32)."""
+    lam = np.geomspace(1.0, kappa, n).astype(np.float32)
+    np.random.default_rng(seed).shuffle(lam)
+    return np.diag(lam).astype(np.float32)
+
+
+def test_block_pcg_one_column_is_pcg():
+    """s = 1: alpha = z.r / p.q and beta = z'.r' / z.r, the single-vector loop."""
+    n = 400
+    A, _ = O.householder_spd(n, 1e4, 1e4, 7)
+    L, rc, _, _, _ = O.factor_mixed(A, nb=64, update_prec=O.FP32)
+    b = np.random.default_rng(2).standard_normal(n)
+    x, it, rr, _ = O.pcg(A, np.tril(L), b, tol=1e-10)
+    X, itb, rel = O.block_pcg(A, np.tril(L), b[:, None], tol=1e-10)
+    assert itb == it and rel[0] <= 1e-10
+    assert np.abs(X[:, 0] - x).max() <= 1e-12 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("n,s", [(32, 4), (32, 8), (64, 8), (64, 16), (32, 32)])
+def test_block_pcg_finite_termination(n, s):
+    """Block CG with s columns spans s new directions per step: on n distinct
+    eigenvalues it terminates within ceil(n/s) steps (one step for s = n),
+    where single-vector CG on the same matrix needs about 20 or more.  A
+    transposed gram or a wrong alpha/beta order loses this property."""
+    A = _diag_spd(n, 4.0, n + s)
+    B = np.random.default_rng(s).standard_normal((n, s))
+    eye = np.eye(n, dtype=np.float32)
+    X, it, rel = O.block_pcg(A, eye, B, tol=1e-10, maxit=200, x0_mode=1)
+    assert 0 < it <= -(-n // s)
+    assert rel.max() <= 1e-12
+    np.testing.assert_allclose(X, B / np.diag(A).astype(np.float64)[:, None], rtol=1e-11, atol=0)
+
+
+def test_block_pcg_matches_fp64_solve():
+    n, s = 500, 6
+    A, _ = O.householder_spd(n, 1e6, 1e6, 11)
+    L, rc, _, _, _ = O.factor_mixed(A, nb=128, update_prec=-1, ladder=(O.E4M3, O.FP32),
+                                    eps_prime=0.5)
+    assert rc == 0
+    B = np.random.default_rng(3).standard_normal((n, s))
+    X, it, rel = O.block_pcg(A, np.tril(L), B, tol=1e-10)
+    assert it > 0 and rel.max() <= 1e-10
+    Xe = np.linalg.solve(A.astype(np.float64), B)
+    assert np.abs(X - Xe).max() / np.abs(Xe).max() < 1e-3
+    # no slower than the independent PCGs it replaces
+    its = [O.pcg(A, np.tril(L), B[:, j], tol=1e-10)[1] for j in range(s)]
+    assert it <= max(its)
+
+
+def test_block_pcg_dependent_columns_break_down():
+    """Two equal right-hand sides make P^T A P singular: the block
+    recurrence reports breakdown (-2) instead of dividing by a zero pivot."""
+    n = 200
+    A, _ = O.householder_spd(n, 1e2, 1e2, 17)
+    B = np.random.default_rng(5).standard_normal((n, 3))
+    B[:, 2] = B[:, 0]
+    X, it, rel = O.block_pcg(A, np.eye(n, dtype=np.float32), B, tol=1e-10, x0_mode=1)
+    assert it == -2 and np.isfinite(X).all()
+
+
+def test_refine_residual_polynomial_and_fixed_point():
+    """x0 = M^-1 b and x += M^-1 r give r_k = (I - A M^-1)^(k+1) b in closed
+    form (a sign error, a transposed solve or a skipped x0 breaks it); the
+    rate is the spectral radius of I - M^-1 A; an exact factor needs one
+    correction; the fixed point is the FP64 solution of the FP32 matrix."""
+    n = 300
+    A, _ = O.householder_spd(n, 1e3, 1e3, 13)
+    A64 = A.astype(np.float64)
+    b = np.random.default_rng(4).standard_normal(n)
+    xe = np.linalg.solve(A64, b)
+    # exact factor rounded to FP32: one correction
+    Lx = np.linalg.cholesky(A64).astype(np.float32)
+    x, it, rr, hist = O.refine(A, Lx, b, tol=1e-12, maxit=10)
+    assert it == 1 and rr <= 1e-12
+    # an E4M3/FP32 switching factor
+    L, rc, _, _, _ = O.factor_mixed(A, nb=64, update_prec=-1, ladder=(O.E4M3, O.FP32),
+                                    eps_prime=0.5)
+    assert rc == 0
+    L = np.tril(L).astype(np.float64)
+    E = np.eye(n) - A64 @ np.linalg.inv(L @ L.T)          # residual propagator
+    rho = np.abs(np.linalg.eigvals(E)).max()
+    assert rho < 0.5
+    x, it, rr, hist = O.refine(A, L.astype(np.float32), b, tol=1e-13, maxit=40)
+    assert it > 1 and rr <= 1e-13
+    r = E @ b
+    for k in range(len(hist)):
+        want = np.linalg.norm(r) / np.linalg.norm(b)
+        if want > 1e-11:                                 # above the rounding floor
+            assert abs(hist[k] - want) <= 1e-6 * want + 1e-13
+        r = E @ r
+    rate = (hist[it - 1] / hist[1]) ** (1.0 / (it - 2)) if it > 2 else hist[1] / hist[0]
+    assert rate <= 1.05 * rho
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-11
+    # maxit reached: -1 and the last residual reported
+    _, it2, rr2, hist2 = O.refine(A, L.astype(np.float32), b, tol=1e-16, maxit=2)
+    assert it2 == -1 and len(hist2) == 3 and rr2 == hist2[-1]
+    # an all-E4M3 factor has rho(I - M^-1 A) > 1 here: refinement diverges
+    # where PCG (whose rate depends on the spread of M^-1 A) still converges
+    L8, rc, _, _, _ = O.factor_mixed(A, nb=64, update_prec=O.E4M3, eps_prime=1.0)
+    L8 = np.tril(L8)
+    E8 = np.eye(n) - A64 @ np.linalg.inv(L8.astype(np.float64) @ L8.T.astype(np.float64))
+    assert np.abs(np.linalg.eigvals(E8)).max() > 1.0
+    assert O.refine(A, L8, b, tol=1e-10, maxit=30)[1] == -1
+    assert O.pcg(A, L8, b, tol=1e-10)[1] > 0
+
+
 def test_factor_breakdown_info():
     A = np.eye(256, dtype=np.float32)
     A[100, 100] = -1
