@@ -286,6 +286,35 @@ def test_pcg_exact_and_iterations():
     assert it3 > 0
 
 
+def test_lanczos_cond_matches_tridiagonal_eigenvalues():
+    """The Lanczos tridiagonal of CG (diagonal 1/a_k + b_{k-1}/a_{k-1},
+    off-diagonal sqrt(b_k)/a_k): Sturm bisection's extreme-eigenvalue ratio
+    equals numpy's on the same matrix, and run to convergence on a small SPD
+    matrix it gives cond(A) to 1e-3."""
+    n = 24
+    lam = np.geomspace(1.0, 50.0, n)
+    A = np.diag(lam).astype(np.float32)
+    b = np.random.default_rng(8).standard_normal(n)
+    _, it, _, _ = O.pcg(A, np.eye(n, dtype=np.float32), b, tol=1e-14, maxit=n, x0_mode=1)
+    al = np.zeros(n + 1)
+    be = np.zeros(n + 1)
+    o = O.PcgOpts(1e-14, n, 0, 1)
+    x = np.zeros(n)
+    rr = O.C.c_double(0)
+    A32 = np.asfortranarray(A)
+    I32 = np.asfortranarray(np.eye(n, dtype=np.float32))
+    k = O.lib().orc_pcg(O._p(A32), n, O._p(I32), n, n, O._p(b), O._p(x), O.C.byref(o), O.C.byref(rr),
+                        O._p(al), O._p(be))
+    k = k if k > 0 else n
+    for m in (4, 10, k):
+        d = np.array([1 / al[i] + (be[i - 1] / al[i - 1] if i else 0.0) for i in range(m)])
+        e = np.array([np.sqrt(be[i]) / al[i] for i in range(m - 1)])
+        T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+        ev = np.linalg.eigvalsh(T)
+        assert abs(O.lib().orc_lanczos_cond(O._p(al), O._p(be), m) - ev[-1] / ev[0]) <= 1e-9 * ev[-1] / ev[0]
+    assert abs(O.lib().orc_lanczos_cond(O._p(al), O._p(be), k) - 50.0) <= 1e-3 * 50.0
+
+
 def _diag_spd(n, kappa, seed):
     """Diagonal SPD with n distinct eigenvalues (exact in FP32)."""
     lam = np.geomspace(1.0, kappa, n).astype(np.float32)
