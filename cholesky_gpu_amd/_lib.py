@@ -21,6 +21,7 @@ ERR_NO_DEVICE = -105
 
 PREC_FP32, PREC_BF16, PREC_FP16, PREC_E5M2, PREC_E4M3, PREC_MXE4M3, PREC_MXE5M2 = range(7)
 PREC_SWITCH = -1
+RULE_MINPIVOT, RULE_FROBENIUS, RULE_SCHUR = 0, 1, 2
 PREC_NAMES = {0: "fp32", 1: "bf16", 2: "fp16", 3: "e5m2", 4: "e4m3", 5: "mxe4m3", 6: "mxe5m2"}
 
 NOALLOC_MAX_TILES = 1024        # MPC_NOALLOC_MAX_TILES of include/mpchol.h
@@ -38,7 +39,7 @@ class MatgenOpts(C.Structure):
 class FactorOpts(C.Structure):
     _fields_ = [("nb", C.c_int), ("update_prec", C.c_int), ("ladder", C.c_int * 4),
                 ("n_ladder", C.c_int), ("eps_prime", C.c_double), ("fallback", C.c_int),
-                ("profile", C.c_int), ("agg_panels", C.c_int)]
+                ("profile", C.c_int), ("agg_panels", C.c_int), ("rule", C.c_int)]
 
 
 class FactorInfo(C.Structure):

@@ -102,9 +102,9 @@ __global__ void init_small_kernel(mpc_small* sm) {
     }
 }
 __global__ void init_factor_kernel(const float* A, int64_t lda, int64_t n, float* tracker,
-                                   uint8_t* map, int64_t nt) {
+                                   float* diag0, uint8_t* map, int64_t nt) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) tracker[i] = A[i + i * lda];
+    if (i < n) tracker[i] = diag0[i] = A[i + i * lda];
     if (map && i < nt * nt) {
         int64_t r = i % nt, c = i / nt;
         map[i] = r == c ? (uint8_t)MPC_PREC_FP32 : (uint8_t)255;
@@ -459,10 +459,13 @@ extern "C" void mpc_factor_opts_default(mpc_factor_opts* o) {
     o->fallback = 1;
     o->profile = 0;
     o->agg_panels = 2;
+    o->rule = MPC_RULE_MINPIVOT;
 }
 
 static int validate_factor_opts(const mpc_factor_opts* o) {
     MPC_CHECK(o->nb == 64 || o->nb == 128 || o->nb == 256, "factor: nb must be 64, 128 or 256");
+    MPC_CHECK(o->rule >= MPC_RULE_MINPIVOT && o->rule <= MPC_RULE_SCHUR,
+              "factor: rule %d is not an MPC_RULE_*", o->rule);
     MPC_CHECK(o->agg_panels >= 1 && o->agg_panels <= MPC_AGG_MAX, "factor: agg_panels must be 1..%d",
               MPC_AGG_MAX);
     if (o->update_prec == MPC_PREC_SWITCH) {
@@ -543,7 +546,8 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     if ((rc = mpc_ws_reserve(h->panel, (size_t)nslot * (n + 128) * nb * 4))) return rc;
     if ((rc = mpc_ws_reserve(h->scales, (size_t)nslot * (n + 128) * (nb / 32)))) return rc;
     if ((rc = mpc_ws_reserve(h->dinv, (size_t)(nt * nb / 32 + 8) * 1024 * sizeof(float)))) return rc;
-    if ((rc = mpc_ws_reserve(h->rowstat, (size_t)(n + 128) * sizeof(float)))) return rc;
+    // row sums of squares, then the original diagonal (switching rule SCHUR)
+    if ((rc = mpc_ws_reserve(h->rowstat, 2 * (size_t)(n + 128) * sizeof(float)))) return rc;
     // SYRK work lists: one region per stream that issues trailing updates
     const int64_t list_ints = mpc_syrk_list_ints(n, MPC_SYRK_TILE);
     if ((rc = mpc_ws_reserve(h->tlist, sizeof(int) * 3 * (size_t)list_ints))) return rc;
@@ -556,6 +560,8 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     uint8_t* scales = (uint8_t*)h->scales.ptr;
     float* dinv = (float*)h->dinv.ptr;
     float* rowss = (float*)h->rowstat.ptr;
+    float* diag0 = rowss + (n + 128);
+    const int rule = o->rule;
     const int ladder[4] = {o->ladder[0], o->ladder[1], o->ladder[2], o->ladder[3]};
     unsigned cmask = 0, fmask = 0;
     mpc_syrk_masks(o->update_prec, ladder, o->n_ladder, &cmask, &fmask);
@@ -564,7 +570,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
     int64_t ninit = std::max<int64_t>(n, nt * nt);
     // (the diagonal of A: from first_src when dA holds only the first columns)
     hipLaunchKernelGGL(init_factor_kernel, dim3((unsigned)((ninit + 255) / 256)), dim3(256), 0, st,
-                       first_src ? first_src : dA, lda, n, tracker, map, nt);
+                       first_src ? first_src : dA, lda, n, tracker, diag0, map, nt);
     MPC_LAUNCH_CHECK();
 
     const int64_t pstride = (int64_t)(n + 128) * nb * 4;     // payload bytes per buffer
@@ -577,19 +583,22 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         float* Ajj = dA + c0 + c0 * lda;
         int r;
         if ((r = mpc_launch_potrf(ps, Ajj, lda, (int)bs, (int)c0, dinv, sm, ntl,
-                                  sm->tile_amax[b], sm->tile_maxrs[b], b)))
+                                  sm->tile_amax[b], sm->tile_maxrs[b], b,
+                                  rule == MPC_RULE_FROBENIUS)))
             return r;
         if (s == 0) return 0;
         float* A21 = dA + e + c0 * lda;
         if ((r = mpc_launch_trsm_stats(ps, A21, lda, s, nb, dinv + (c0 / 32) * 1024, Ajj,
                                        rowss + e, tracker + e, tr, sm, sm->tile_amax[b],
-                                       sm->tile_maxrs[b])))
+                                       sm->tile_maxrs[b], 0, -1,
+                                       rule == MPC_RULE_SCHUR ? diag0 + e : nullptr, b)))
             return r;
         return mpc_launch_quantize_ex(ps, A21, s, nb, lda, tr, nullptr, sm->tile_amax[b],
                                       sm->tile_maxrs[b], sm, o->update_prec, ladder, o->n_ladder,
                                       eps, sm->tile_prec[q], map ? map + (j + 1) + j * nt : nullptr,
                                       payload + q * pstride, (int64_t)nb * 4, scales + q * sstride,
-                                      &sm->nan_flag, 0, -1, b, tr / nb, nt - (j + 1));
+                                      &sm->nan_flag, 0, -1, b, tr / nb, nt - (j + 1), rule,
+                                      rowss + e);
     };
     // trailing update of panel j on stream us restricted to SYRK tile columns
     // [t0, t1) of A22 (tiles of MPC_SYRK_TILE, or of 64 * mixed when mixed)
@@ -743,13 +752,15 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
             float* A21 = dA + e + c0 * lda;
             MPC_HIP(hipEventRecord(ev[0], st));
             if ((rc = mpc_launch_potrf(st, Ajj, lda, (int)bs, (int)c0, dinv, sm, ntl,
-                                       sm->tile_amax[b], sm->tile_maxrs[b], b)))
+                                       sm->tile_amax[b], sm->tile_maxrs[b], b,
+                                       rule == MPC_RULE_FROBENIUS)))
                 return rc;
             MPC_HIP(hipEventRecord(ev[1], st));
             if (s > 0) {
                 if ((rc = mpc_launch_trsm_stats(st, A21, lda, s, nb, dinv + (c0 / 32) * 1024, Ajj,
                                                 rowss + e, tracker + e, tr, sm, sm->tile_amax[b],
-                                                sm->tile_maxrs[b])))
+                                                sm->tile_maxrs[b], 0, -1,
+                                                rule == MPC_RULE_SCHUR ? diag0 + e : nullptr, b)))
                     return rc;
                 MPC_HIP(hipEventRecord(ev[2], st));
                 if ((rc = mpc_launch_quantize_ex(
@@ -757,7 +768,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                          o->update_prec, ladder, o->n_ladder, eps, sm->tile_prec[b],
                          map ? map + (j + 1) + j * nt : nullptr, payload + b * pstride,
                          (int64_t)nb * 4, scales + b * sstride, &sm->nan_flag, 0, -1, b, tr / nb,
-                         nt - (j + 1))))
+                         nt - (j + 1), rule, rowss + e)))
                     return rc;
                 MPC_HIP(hipEventRecord(ev[3], st));
                 if ((rc = update(j, 0, -1, st, 0))) return rc;

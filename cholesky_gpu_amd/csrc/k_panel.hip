@@ -46,7 +46,9 @@ __global__ __launch_bounds__(NW * 64, 2) void trsm_stats_kernel(float* __restric
                                                          mpc_small* __restrict__ sm,
                                                          uint32_t* __restrict__ t_amax,
                                                          uint32_t* __restrict__ t_maxrs,
-                                                         int64_t row0) {
+                                                         int64_t row0,
+                                                         const float* __restrict__ diag0,
+                                                         int slot) {
     constexpr int NO = NBK / NW;          // blocks per wave
     constexpr int XLD = 33;
     __shared__ float Xs[NBK][32 * XLD];
@@ -149,6 +151,18 @@ __global__ __launch_bounds__(NW * 64, 2) void trsm_stats_kernel(float* __restric
         rowss[row] = tss;
         tracker[row] -= tss;
     }
+    // switching rule SCHUR: largest tracker / original diagonal below the panel
+    if (diag0) {
+        float q = 0.f;
+        if (valid && h == 0 && wave == 0) {
+            const float d0 = diag0[row];
+            const float qq = tracker[row] / d0;
+            q = qq > 0.f ? qq : 0.f;            // NaN and negative ratios count as 0
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) q = fmaxf(q, __shfl_xor(q, o));
+        if (lane == 0 && wave == 0) atomicMax(&sm->schur_max[slot], __float_as_uint(q));
+    }
     if (!(tss == tss) || !(tam <= 3.4e38f)) atomicOr(&sm->nan_flag, 1);
     // slab max (all 32 rows of the slab are in one tile: tile_rows % 32 == 0)
     float wss = tss, wam = tam;
@@ -190,15 +204,17 @@ struct rule_args {
     int n_ladder;
     int ladder[4];
     double eps_prime;
+    int rule;                 // MPC_RULE_*
 };
 
-// FP16 is stored unscaled: a tile with an entry above its largest finite
-// value (65504) takes the next format of the ladder
-__device__ __forceinline__ int decide(const rule_args& ra, float maxrs, float mu, float amax) {
+// The lowest ladder format p with u_p * stat <= eps' * ref (stat and ref as
+// the rule defines them, include/mpchol.h).  FP16 is stored unscaled: a tile
+// with an entry above its largest finite value (65504) takes the next format.
+__device__ __forceinline__ int decide(const rule_args& ra, float stat, float ref, float amax) {
     if (ra.update_prec >= 0) return ra.update_prec;
     for (int q = 0; q < ra.n_ladder; ++q) {
         if (ra.ladder[q] == MPC_PREC_FP16 && !(amax <= 65504.f)) continue;
-        if (mpc_dev_unit_roundoff(ra.ladder[q]) * (double)maxrs <= ra.eps_prime * (double)mu)
+        if (mpc_dev_unit_roundoff(ra.ladder[q]) * (double)stat <= ra.eps_prime * (double)ref)
             return ra.ladder[q];
     }
     return ra.ladder[ra.n_ladder - 1];
@@ -211,7 +227,7 @@ __global__ __launch_bounds__(256) void quantize_kernel(
     const uint32_t* __restrict__ tile_maxrs, const mpc_small* __restrict__ smc, rule_args ra,
     uint8_t* __restrict__ prec_out, uint8_t* __restrict__ map_col, uint8_t* __restrict__ payload,
     int64_t row_stride, uint8_t* __restrict__ scales, int* __restrict__ nan_flag, int64_t row0,
-    int mu_slot, int map_rep, int64_t map_len) {
+    int mu_slot, int map_rep, int64_t map_len, const float* __restrict__ rowss) {
     if (smc && smc->info != 0) return;
     const int64_t r0 = row0 + (int64_t)blockIdx.x * 32;
     const int r = threadIdx.x & 31, qq0 = threadIdx.x >> 5;
@@ -221,8 +237,31 @@ __global__ __launch_bounds__(256) void quantize_kernel(
     if (prec_in) {
         prec = prec_in[tile];
     } else {
-        float mu = __uint_as_float(smc->mu_step[mu_slot]);
-        prec = decide(ra, __uint_as_float(tile_maxrs[tile]), mu, __uint_as_float(tile_amax[tile]));
+        float stat, ref;
+        if (ra.rule == MPC_RULE_FROBENIUS) {
+            // ||tile||_F^2: the tile's row sums of squares added in row order
+            // (every workgroup of the tile forms the same sum)
+            __shared__ float fr[256];
+            const int64_t t0 = (int64_t)tile * tile_rows;
+            const int nr = (int)(m - t0 < tile_rows ? m - t0 : tile_rows);
+            float v = 0.f;
+            for (int q = threadIdx.x; q < nr; q += 256) v += rowss[t0 + q];
+            fr[threadIdx.x] = v;
+            __syncthreads();
+            for (int w = 128; w > 0; w >>= 1) {
+                if ((int)threadIdx.x < w) fr[threadIdx.x] += fr[threadIdx.x + w];
+                __syncthreads();
+            }
+            stat = fr[0];
+            ref = smc->diag_f2[mu_slot];
+        } else if (ra.rule == MPC_RULE_SCHUR) {
+            stat = __uint_as_float(smc->schur_max[mu_slot]);
+            ref = 1.f;
+        } else {
+            stat = __uint_as_float(tile_maxrs[tile]);
+            ref = __uint_as_float(smc->mu_step[mu_slot]);
+        }
+        prec = decide(ra, stat, ref, __uint_as_float(tile_amax[tile]));
         if (r0 % tile_rows == 0 && threadIdx.x == 0) {
             if (prec_out) prec_out[tile] = (uint8_t)prec;
             // the map is in nb blocks: a tile of map_rep blocks fills each of them
@@ -300,7 +339,7 @@ __global__ __launch_bounds__(256) void quantize_kernel(
 int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, int nb,
                           const float* dinv_j, const float* Ljj, float* rowss, float* tracker,
                           int tile_rows, mpc_small* sm, uint32_t* tile_amax, uint32_t* tile_maxrs,
-                          int64_t row0, int64_t row1) {
+                          int64_t row0, int64_t row1, const float* diag0, int slot) {
     if (row1 < 0 || row1 > s) row1 = s;
     if (row1 <= row0) return 0;
     if (tile_rows % 32 || row0 % 32) {
@@ -319,17 +358,17 @@ int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, in
     case 64:
         hipLaunchKernelGGL((trsm_stats_kernel<2, 2, kTrsmRefine>), grid, dim3(128), 0, st, A21,
                            lda, s, dinv_j, Ljj, rowss, tracker, tile_rows, sm, tile_amax,
-                           tile_maxrs, row0);
+                           tile_maxrs, row0, diag0, slot & 1);
         break;
     case 128:
         hipLaunchKernelGGL((trsm_stats_kernel<4, 4, kTrsmRefine>), grid, dim3(256), 0, st, A21,
                            lda, s, dinv_j, Ljj, rowss, tracker, tile_rows, sm, tile_amax,
-                           tile_maxrs, row0);
+                           tile_maxrs, row0, diag0, slot & 1);
         break;
     case 256:
         hipLaunchKernelGGL((trsm_stats_kernel<8, 4, kTrsmRefine>), grid, dim3(256), 0, st, A21,
                            lda, s, dinv_j, Ljj, rowss, tracker, tile_rows, sm, tile_amax,
-                           tile_maxrs, row0);
+                           tile_maxrs, row0, diag0, slot & 1);
         break;
     default:
         mpc_set_error("trsm: unsupported block size %d", nb);
@@ -355,7 +394,7 @@ int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int
                            const int* ladder, int n_ladder, double eps_prime, uint8_t* prec_out,
                            uint8_t* map_col, uint8_t* payload, int64_t row_stride, uint8_t* scales,
                            int* nan_flag, int64_t row0, int64_t row1, int mu_slot, int map_rep,
-                           int64_t map_len) {
+                           int64_t map_len, int rule, const float* rowss) {
     if (map_len < 0) map_len = ((m + tile_rows - 1) / tile_rows) * (int64_t)map_rep;
     if (row1 < 0 || row1 > m) row1 = m;
     if (row1 <= row0) return 0;
@@ -369,10 +408,15 @@ int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int
     ra.n_ladder = n_ladder;
     for (int i = 0; i < 4; ++i) ra.ladder[i] = i < n_ladder ? ladder[i] : 0;
     ra.eps_prime = eps_prime;
+    ra.rule = rule;
+    if (!prec_in && update_prec < 0 && rule == MPC_RULE_FROBENIUS && !rowss) {
+        mpc_set_error("quantize: the Frobenius rule needs the row sums of squares");
+        return MPC_ERR_ARG;
+    }
     hipLaunchKernelGGL(quantize_kernel, dim3((unsigned)((row1 - row0 + 31) / 32)), dim3(256), 0, st,
                        P, m, k, ld, tile_rows, prec_in, tile_amax, tile_maxrs, sm, ra, prec_out,
                        map_col, payload, row_stride, scales, nan_flag, row0, mu_slot & 1, map_rep,
-                       map_len);
+                       map_len, rowss);
     MPC_LAUNCH_CHECK();
     return 0;
 }

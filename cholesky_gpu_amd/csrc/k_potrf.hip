@@ -234,7 +234,7 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
                                                     mpc_small* __restrict__ sm, int ntiles_next,
                                                     uint32_t* __restrict__ st_amax,
                                                     uint32_t* __restrict__ st_maxrs,
-                                                    int mu_slot) {
+                                                    int mu_slot, int want_f2) {
     __shared__ float T[NSUB * SUBSZ];
     __shared__ float Dinv[SUBSZ];
     __shared__ __align__(16) float colb[2][32];     // column k of L by row parity
@@ -531,6 +531,28 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
             const uint32_t before = atomicMin(&sm->mu_bits, mine);
             sm->mu_step[mu_slot] = before < mine ? before : mine;
         }
+        sm->schur_max[mu_slot] = 0u;     // the step's TRSM folds its ratios in
+    }
+    if (want_f2) {
+        // ||L_jj||_F^2: thread t sums the lower-triangle entries (r, c),
+        // r >= c, of sub-tiles (ib, jb), jb <= ib, at element index
+        // t, t + NT, ... of each sub-tile in order; then a fixed-order tree
+        mpc_lds_barrier();
+        float f2 = 0.f;
+        for (int ib = 0; ib < nsb; ++ib)
+            for (int jb = 0; jb <= ib; ++jb)
+                for (int e = tid; e < 1024; e += NT) {
+                    const int r = ib * 32 + (e & 31), c = jb * 32 + (e >> 5);
+                    const float v = T[sidx(ib, jb) * SUBSZ + (e & 31) + (e >> 5) * SUBLD];
+                    f2 = (r < bs && c <= r) ? fmaf(v, v, f2) : f2;
+                }
+        if (tid < 256) red[tid] = f2;
+        mpc_lds_barrier();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (tid < w) red[tid] += red[tid + w];
+            mpc_lds_barrier();
+        }
+        if (tid == 0) sm->diag_f2[mu_slot] = red[0];
     }
 }
 
@@ -538,7 +560,7 @@ __global__ __launch_bounds__(NT) void potrf_kernel(float* __restrict__ A, int64_
 
 int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0, float* dinv,
                      mpc_small* sm, int ntiles_next, uint32_t* stats_amax, uint32_t* stats_maxrs,
-                     int mu_slot) {
+                     int mu_slot, int want_f2) {
     if (bs <= 0 || bs > 256) {
         mpc_set_error("potrf: block size %d outside 1..256", bs);
         return MPC_ERR_ARG;
@@ -556,7 +578,7 @@ int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0, fl
         return MPC_ERR_ARG;
     }
     hipLaunchKernelGGL(potrf_kernel, dim3(1), dim3(NT), 0, st, A, lda, bs, col0, dinv, sm,
-                       ntiles_next, stats_amax, stats_maxrs, mu_slot);
+                       ntiles_next, stats_amax, stats_maxrs, mu_slot, want_f2 ? 1 : 0);
     MPC_LAUNCH_CHECK();
     return 0;
 }

@@ -165,6 +165,11 @@ struct mpc_small {
     // decisions of that step's panel read this copy, which stays fixed while
     // the next step's POTRF lowers mu_bits
     uint32_t mu_step[2];
+    // by step parity too: ||L_jj||_F^2 of the step's diagonal block (POTRF)
+    // and the largest ratio tracker / original diagonal below it (TRSM;
+    // bits of a non-negative float, atomicMax), for the switching rules
+    float    diag_f2[2];
+    uint32_t schur_max[2];
     uint32_t pad;
     // per-step tile stats (index by tile within the panel), max 1024 tiles,
     // double-buffered by step parity so panel j+1 can be prepared while the
@@ -183,21 +188,23 @@ static_assert(sizeof(((mpc_small*)nullptr)->tile_prec) / 1024 == MPC_PANEL_SLOTS
 // k_potrf.hip
 int mpc_launch_potrf(hipStream_t st, float* A, int64_t lda, int bs, int col0,
                      float* dinv, mpc_small* sm, int ntiles_next, uint32_t* stats_amax,
-                     uint32_t* stats_maxrs, int mu_slot = 0);
+                     uint32_t* stats_maxrs, int mu_slot = 0, int want_f2 = 0);
 // k_panel.hip
 // rows [row0, row1) of the s-row panel (row1 < 0: s); the panel kernels
 // index tiles and statistics by the row within the whole panel
 int mpc_launch_trsm_stats(hipStream_t st, float* A21, int64_t lda, int64_t s, int nb,
                           const float* dinv_j, const float* Ljj, float* rowss, float* tracker,
                           int tile_rows, mpc_small* sm, uint32_t* tile_amax, uint32_t* tile_maxrs,
-                          int64_t row0 = 0, int64_t row1 = -1);
+                          int64_t row0 = 0, int64_t row1 = -1, const float* diag0 = nullptr,
+                          int slot = 0);
 int mpc_launch_quantize_ex(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
                            int tile_rows, const uint8_t* prec_in, const uint32_t* tile_amax,
                            const uint32_t* tile_maxrs, mpc_small* sm, int update_prec,
                            const int* ladder, int n_ladder, double eps_prime, uint8_t* prec_out,
                            uint8_t* map_col, uint8_t* payload, int64_t row_stride, uint8_t* scales,
                            int* nan_flag, int64_t row0 = 0, int64_t row1 = -1, int mu_slot = 0,
-                           int map_rep = 1, int64_t map_len = -1);
+                           int map_rep = 1, int64_t map_len = -1, int rule = MPC_RULE_MINPIVOT,
+                           const float* rowss = nullptr);
 int mpc_launch_tile_amax(hipStream_t st, const float* P, int64_t m, int k, int64_t ld,
                          int tile_rows, uint32_t* tile_amax);
 // k_syrk.hip

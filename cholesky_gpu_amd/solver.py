@@ -62,9 +62,10 @@ def _pre(h):
 
 # ---------------------------------------------------------------- options
 def factor_opts(nb=256, update_prec=L.PREC_SWITCH, ladder=(L.PREC_E4M3, L.PREC_FP32),
-                eps_prime=0.25, fallback=1, profile=0, agg_panels=None):
+                eps_prime=0.25, fallback=1, profile=0, agg_panels=None, rule=L.RULE_MINPIVOT):
     o = L.FactorOpts()
     L.load().mpc_factor_opts_default(C.byref(o))
+    o.rule = rule
     o.nb = nb
     o.update_prec = update_prec
     for i in range(4):

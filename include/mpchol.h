@@ -130,11 +130,27 @@ typedef struct {
     int    agg_panels;      /* panels whose trailing updates beyond their group
                                of block columns are applied together
                                (K = agg_panels * nb): 1..4, default 2 */
+    int    rule;            /* MPC_RULE_* below (default MPC_RULE_MINPIVOT) */
 } mpc_factor_opts;
-/* Rule (per panel tile L_ij, i > j): the lowest-precision ladder entry p with
-       u_p * max_{r in tile} sum_k L_rk^2  <=  eps_prime * min_k L_kk^2,
-   the minimum taken over the diagonal blocks factored so far.  A tile pair
-   (i, l) of the trailing update is computed in the higher of the two. */
+/* Switching rules.  Each picks, for a panel tile L_ij (i > j) of panel j, the
+   lowest-precision ladder entry p (unit roundoff u_p) with
+       u_p * stat  <=  eps_prime * ref,
+   FP16 skipped for a tile with an entry above 65504; the last ladder entry
+   when none qualifies.  A tile pair (i, l) of the trailing update is computed
+   in the higher of the two formats.
+     MPC_RULE_MINPIVOT (0): stat = max over the tile's rows r of sum_k L_rk^2
+       (k over panel j), ref = min_k L_kk^2 over the diagonal blocks factored so
+       far.
+     MPC_RULE_FROBENIUS (1): stat = ||L_ij||_F^2, ref = ||L_jj||_F^2 (the tile's
+       Frobenius norm relative to its diagonal block).
+     MPC_RULE_SCHUR (2): stat = max over the rows r below the panel of
+       d_r / a_rr, ref = 1, one decision for every tile of the panel, where d_r
+       is the Schur-diagonal tracker after panel j (a_rr - sum_{k<=j blocks}
+       L_rk^2) and a_rr the original diagonal: the per-step test of can_switch
+       (kernels_macros.cuh:191-211) evaluated deterministically. */
+#define MPC_RULE_MINPIVOT  0
+#define MPC_RULE_FROBENIUS 1
+#define MPC_RULE_SCHUR     2
 
 typedef struct {
     int    info;            /* 0, or 1-based column of the first bad pivot */

@@ -24,6 +24,11 @@
  *   D3 Precision is chosen per panel tile by the min-pivot rule
  *      (orc_rule_minpivot) instead of can_switch's racy per-step atomicExch
  *      (kernels_macros.cuh:191-211, read on the host at switching_chol.cuh:124).
+ *      Two other rules are options: the tile's Frobenius norm against its
+ *      diagonal block's (rule 1), and can_switch's own test evaluated
+ *      deterministically for the whole panel (rule 2): the lowest format p
+ *      with u_p * max_i d_i / a_ii <= eps' over the rows below the panel,
+ *      d_i the Schur-diagonal tracker, a_ii the original diagonal.
  *      FP16, stored unscaled, is skipped for a tile whose largest magnitude
  *      exceeds FP16's largest finite value 65504 (the next ladder format is
  *      taken instead).
@@ -759,6 +764,8 @@ int orc_factor_mixed(float* A, int64_t n, int64_t lda, const orc_factor_opts* o,
     float* dloc = NULL;
     if (!d) d = dloc = (float*)malloc(sizeof(float) * n);
     for (int64_t i = 0; i < n; ++i) d[i] = A[i + i * lda];
+    float* d0 = (float*)malloc(sizeof(float) * n);       /* original diagonal (rule 2) */
+    memcpy(d0, d, sizeof(float) * n);
     float* Q = (float*)malloc(sizeof(float) * n * nb);
     uint8_t* pay = (uint8_t*)malloc((size_t)n * nb * 4);
     uint8_t* sc = (uint8_t*)malloc((size_t)n * (nb / 32 > 0 ? nb / 32 : 1));
@@ -802,6 +809,17 @@ int orc_factor_mixed(float* A, int64_t n, int64_t lda, const orc_factor_opts* o,
             rs[i] = acc;
         }
         int64_t ntl = (s + tr - 1) / tr;
+        /* reference values of the switching rules (D3) */
+        double f2jj = 0.0, schur = 0.0;
+        if (o->rule == 1) {
+            for (int64_t c = 0; c < bs; ++c)
+                for (int64_t r = c; r < bs; ++r) f2jj += (double)Ajj[r + c * lda] * (double)Ajj[r + c * lda];
+        } else if (o->rule == 2) {
+            for (int64_t i = e; i < n; ++i) {
+                const float q = d[i] / d0[i];
+                if (q > schur) schur = q;               /* NaN and negative ratios count as 0 */
+            }
+        }
         for (int64_t t = 0; t < ntl; ++t) {
             int p;
             if (map_in) {
@@ -817,7 +835,15 @@ int orc_factor_mixed(float* A, int64_t n, int64_t lda, const orc_factor_opts* o,
                         if (v > am) am = v;
                     }
                 }
-                p = orc_rule_minpivot_amax(o->ladder, o->n_ladder, o->eps_prime, mx, mu, am);
+                if (o->rule == 1) {                     /* ||tile||_F^2 vs ||L_jj||_F^2 */
+                    double f2 = 0.0;
+                    for (int64_t i = e + t * tr; i < e + (t + 1) * tr && i < n; ++i) f2 += rs[i];
+                    p = orc_rule_minpivot_amax(o->ladder, o->n_ladder, o->eps_prime, f2, f2jj, am);
+                } else if (o->rule == 2) {              /* max d_i / a_ii vs 1 */
+                    p = orc_rule_minpivot_amax(o->ladder, o->n_ladder, o->eps_prime, schur, 1.0, am);
+                } else {
+                    p = orc_rule_minpivot_amax(o->ladder, o->n_ladder, o->eps_prime, mx, mu, am);
+                }
             }
             ptile[t] = (uint8_t)p;
             /* the map is in nb blocks: a tile of tr / nb blocks fills each */
@@ -844,7 +870,7 @@ int orc_factor_mixed(float* A, int64_t n, int64_t lda, const orc_factor_opts* o,
     inf.info = ret > 0 ? ret : 0;
     inf.min_pivot = mu;
     if (info) *info = inf;
-    free(Q); free(pay); free(sc); free(ptile); free(rs); free(dloc);
+    free(Q); free(pay); free(sc); free(ptile); free(rs); free(dloc); free(d0);
     return ret;
 }
 

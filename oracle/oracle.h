@@ -133,6 +133,7 @@ typedef struct {
     int    n_ladder;
     double eps_prime;     /* switching threshold */
     int    tile_rows;     /* rows per precision tile (= nb) */
+    int    rule;          /* 0 min pivot, 1 Frobenius, 2 Schur diagonal (D3) */
 } orc_factor_opts;
 
 typedef struct {

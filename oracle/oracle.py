@@ -57,7 +57,8 @@ D = C.c_double
 
 class FactorOpts(C.Structure):
     _fields_ = [("nb", C.c_int), ("update_prec", C.c_int), ("ladder", C.c_int * 4),
-                ("n_ladder", C.c_int), ("eps_prime", C.c_double), ("tile_rows", C.c_int)]
+                ("n_ladder", C.c_int), ("eps_prime", C.c_double), ("tile_rows", C.c_int),
+                ("rule", C.c_int)]
 
 
 class FactorInfo(C.Structure):
@@ -350,8 +351,11 @@ def symv(A, x):
 
 
 # ---------------------------------------------------------------- reference path
+RULE_MINPIVOT, RULE_FROBENIUS, RULE_SCHUR = 0, 1, 2
+
+
 def factor_mixed(A, nb=256, update_prec=FP32, ladder=(E4M3, FP32), eps_prime=1.0,
-                 map_in=None, max_steps=-1, tile_rows=None):
+                 map_in=None, max_steps=-1, tile_rows=None, rule=RULE_MINPIVOT):
     """Returns (L [full array, lower valid], info, map, updated_diag, FactorInfo)."""
     L = np.array(A, dtype=np.float32, order="F", copy=True)
     n = L.shape[0]
@@ -363,6 +367,7 @@ def factor_mixed(A, nb=256, update_prec=FP32, ladder=(E4M3, FP32), eps_prime=1.0
     o.n_ladder = len(ladder)
     o.eps_prime = eps_prime
     o.tile_rows = tile_rows or nb
+    o.rule = rule
     nt = (n + nb - 1) // nb
     mp = np.full((nt, nt), 255, np.uint8, order="F")
     ud = np.zeros(n, np.float32)

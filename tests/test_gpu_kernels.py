@@ -268,30 +268,58 @@ def hh(oracle, n, seed=11):
     return A
 
 
-@pytest.mark.parametrize("n,nb,prec,ladder,agg", [
-    (1024, 256, L.PREC_FP32, (L.PREC_FP32,), 4),
-    (1024, 256, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32), 4),
-    (1100, 128, L.PREC_SWITCH, (L.PREC_MXE4M3, L.PREC_BF16, L.PREC_FP32), 4),
-    (2048, 256, L.PREC_SWITCH, (L.PREC_E5M2, L.PREC_FP32), 4),
-    (700, 64, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32), 4),
+@pytest.mark.parametrize("n,nb,prec,ladder,agg,rule", [
+    (1024, 256, L.PREC_FP32, (L.PREC_FP32,), 4, 0),
+    (1024, 256, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32), 4, 0),
+    (1100, 128, L.PREC_SWITCH, (L.PREC_MXE4M3, L.PREC_BF16, L.PREC_FP32), 4, 0),
+    (2048, 256, L.PREC_SWITCH, (L.PREC_E5M2, L.PREC_FP32), 4, 0),
+    (700, 64, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32), 4, 0),
     # updates per panel (no aggregation), and groups of 3 (a ragged last group)
-    (2048, 256, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32), 1),
-    (2560, 256, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32), 3),
-    (1100, 128, L.PREC_SWITCH, (L.PREC_MXE4M3, L.PREC_BF16, L.PREC_FP32), 2),
+    (2048, 256, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32), 1, 0),
+    (2560, 256, L.PREC_SWITCH, (L.PREC_E4M3, L.PREC_FP32), 3, 0),
+    (1100, 128, L.PREC_SWITCH, (L.PREC_MXE4M3, L.PREC_BF16, L.PREC_FP32), 2, 0),
 ])
-def test_factor_matches_oracle(handle, oracle, n, nb, prec, ladder, agg):
-    say("factor", n, nb, prec, ladder, agg)
-    A = hh(oracle, n)
+def test_factor_matches_oracle(handle, oracle, n, nb, prec, ladder, agg, rule):
+    say("factor", n, nb, prec, ladder, agg, rule)
+    factor_parity(handle, oracle, hh(oracle, n), nb, prec, ladder, agg, rule, 0.25)
+
+
+def lowrank_spd(n, rank, seed):
+    """I + V V^T with |v_i|^2 ~ 64: the Schur diagonal stays near a_ii until
+    the first rank rows are eliminated and then drops to about 1/65 of it."""
+    V = np.random.default_rng(seed).standard_normal((n, rank)) * (8 / np.sqrt(rank))
+    return (np.eye(n) + V @ V.T).astype(np.float32)
+
+
+@pytest.mark.parametrize("case,nb,ladder,rule,eps", [
+    ("hh1536", 256, (L.PREC_E4M3, L.PREC_FP32), L.RULE_FROBENIUS, 0.01),
+    ("hh1100", 128, (L.PREC_E5M2, L.PREC_FP32), L.RULE_FROBENIUS, 0.05),
+    ("lowrank1536", 256, (L.PREC_E4M3, L.PREC_FP32), L.RULE_SCHUR, 0.003),
+])
+def test_factor_rules_match_oracle(handle, oracle, case, nb, ladder, rule, eps):
+    """The tile-Frobenius and Schur-diagonal rules against the oracle, on
+    matrices and thresholds where they give a mixed schedule and the factor
+    succeeds (both rules ignore the pivots and break down on these matrices
+    at eps' = 0.25, in the oracle as on the GPU)."""
+    say("factor rule", case, rule, eps)
+    A = lowrank_spd(1536, 384, 3) if case.startswith("lowrank") else hh(oracle, int(case[2:]))
+    gmap = factor_parity(handle, oracle, A, nb, L.PREC_SWITCH, ladder, 2, rule, eps)
+    below = np.tril(np.ones_like(gmap, bool), -1)
+    assert len(np.unique(gmap[below])) == 2          # a mixed schedule
+
+
+def factor_parity(handle, oracle, A, nb, prec, ladder, agg, rule, eps):
+    n = A.shape[0]
     T = to_dev_cm(A)
-    opts = S.factor_opts(nb=nb, update_prec=prec, ladder=ladder, eps_prime=0.25, fallback=0,
-                         agg_panels=agg)
+    opts = S.factor_opts(nb=nb, update_prec=prec, ladder=ladder, eps_prime=eps, fallback=0,
+                         agg_panels=agg, rule=rule)
     rc, fi, mp, dg = S.factor(handle, T, opts, want_map=True, want_diag=True)
     assert rc == 0, rc
     Lg = np.tril(from_dev_cm(T))
     gmap = mp.cpu().numpy().T          # gmap[i, j] = precision of tile L_ij
     # oracle on the GPU's schedule (threshold flips would otherwise differ)
     Lo, rco, omap, udo, ofi = oracle.factor_mixed(A, nb=nb, update_prec=prec, ladder=ladder,
-                                                  eps_prime=0.25, map_in=gmap,
+                                                  eps_prime=eps, map_in=gmap,
                                                   tile_rows=max(nb, 128))
     assert rco == 0
     # the map holds one entry per nb block of L below the diagonal, also when
@@ -312,7 +340,7 @@ def test_factor_matches_oracle(handle, oracle, n, nb, prec, ladder, agg):
     assert bg < 2 * bo + 1e-6, (bg, bo)
     # the oracle's own rule gives (nearly) the same schedule
     _, _, omap2, _, _ = oracle.factor_mixed(A, nb=nb, update_prec=prec, ladder=ladder,
-                                            eps_prime=0.25, tile_rows=max(nb, 128))
+                                            eps_prime=eps, tile_rows=max(nb, 128), rule=rule)
     low = np.tril(np.ones_like(gmap, bool), -1) & (omap2 != 255)
     f = max(nb, 128) // nb                    # map entries per row tile
     flips = np.sum(gmap[low] != omap2[low]) // f
@@ -324,6 +352,34 @@ def test_factor_matches_oracle(handle, oracle, n, nb, prec, ladder, agg):
     for c in range(8):
         assert abs(fi.syrk_flops[c] - ofi.flops[c]) <= 1e-9 * sum(ofi.flops), (c, list(fi.syrk_flops),
                                                                               list(ofi.flops))
+    return gmap
+
+
+@pytest.mark.parametrize("rule,eps_hi,eps_lo", [
+    (L.RULE_MINPIVOT, 1.05, 0.95), (L.RULE_FROBENIUS, 1.05, 0.95), (L.RULE_SCHUR, 0.04, 0.03),
+])
+def test_switching_rules_closed_form(handle, oracle, rule, eps_hi, eps_lo):
+    """A = L L^T with L = [[2I, 0], [B, 8I]], B = +-1/2 (256-blocks): each rule's
+    threshold falls where its closed form puts it (u_E4M3 = 1/16), on the GPU
+    as in the oracle, and the factor is exact in both formats."""
+    m = 256
+    B = np.where(np.random.default_rng(12).random((m, m)) < 0.5, -0.5, 0.5)
+    Lx = np.zeros((2 * m, 2 * m))
+    Lx[:m, :m] = 2 * np.eye(m)
+    Lx[m:, :m] = B
+    Lx[m:, m:] = 8 * np.eye(m)
+    A = (Lx @ Lx.T).astype(np.float32)
+    for eps, want in ((eps_hi, L.PREC_E4M3), (eps_lo, L.PREC_FP32)):
+        say("rule", rule, eps)
+        T = to_dev_cm(A)
+        opts = S.factor_opts(nb=256, update_prec=L.PREC_SWITCH, ladder=(L.PREC_E4M3, L.PREC_FP32),
+                             eps_prime=eps, fallback=0, rule=rule)
+        rc, fi, mp, dg = S.factor(handle, T, opts, want_map=True, want_diag=True)
+        assert rc == 0
+        gmap = mp.cpu().numpy().T
+        assert gmap[1, 0] == want, (rule, eps, gmap)
+        np.testing.assert_allclose(np.tril(from_dev_cm(T)), Lx, rtol=0, atol=4e-6)
+        np.testing.assert_allclose(dg.cpu().numpy()[m:], 64.0, rtol=0, atol=1e-4)
 
 
 @pytest.mark.parametrize("prec,agg", [

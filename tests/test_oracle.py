@@ -422,6 +422,39 @@ def test_refine_residual_polynomial_and_fixed_point():
     assert O.pcg(A, L8, b, tol=1e-10)[1] > 0
 
 
+def _rules_case(c=8.0):
+    """A = L L^T for L = [[2 I, 0], [B, c I]] (256 x 256 blocks, B = +-1/2):
+    every step of the factor is exact in FP32.  Panel 0's tile (256 rows)
+    has row sums of squares 64, Frobenius^2 256 * 64, L_11 = 2 I
+    (Frobenius^2 1024, pivots^2 4), and the Schur tracker below it is c^2
+    against the original diagonal 64 + c^2."""
+    m = 256
+    B = np.where(np.random.default_rng(12).random((m, m)) < 0.5, -0.5, 0.5)
+    Lx = np.zeros((2 * m, 2 * m))
+    Lx[:m, :m] = 2 * np.eye(m)
+    Lx[m:, :m] = B
+    Lx[m:, m:] = c * np.eye(m)
+    return (Lx @ Lx.T).astype(np.float32), Lx
+
+
+@pytest.mark.parametrize("rule,eps_hi,eps_lo", [
+    (O.RULE_MINPIVOT, 1.05, 0.95),       # u * 64 <= eps * 4
+    (O.RULE_FROBENIUS, 1.05, 0.95),      # u * 16384 <= eps * 1024
+    (O.RULE_SCHUR, 0.04, 0.03),          # u * 64 / 128 <= eps
+])
+def test_switching_rules_closed_form(rule, eps_hi, eps_lo):
+    """Each rule's threshold lands where its closed form says (u_E4M3 = 1/16):
+    E4M3 just above it, FP32 just below; the factor is exact either way."""
+    A, Lx = _rules_case()
+    for eps, want in ((eps_hi, O.E4M3), (eps_lo, O.FP32)):
+        L, rc, mp, ud, _ = O.factor_mixed(A, nb=256, update_prec=-1, ladder=(O.E4M3, O.FP32),
+                                          eps_prime=eps, rule=rule)
+        assert rc == 0 and mp[1, 0] == want, (rule, eps, mp[1, 0])
+        if want == O.FP32:
+            np.testing.assert_array_equal(np.tril(L), Lx.astype(np.float32))
+        np.testing.assert_array_equal(ud[256:], np.float32(64.0))
+
+
 def test_factor_breakdown_info():
     A = np.eye(256, dtype=np.float32)
     A[100, 100] = -1
