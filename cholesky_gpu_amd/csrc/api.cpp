@@ -671,19 +671,29 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
         if ((rc = panel(0, ps))) return rc;
         if ((rc = build_w(0, ps))) return rc;
         bool far_pending = false;
+        // ps has waited for st's last recorded update: a wait on a queue
+        // costs several microseconds of the chain even when the event is
+        // long complete, so ps waits only for st work it has not seen
+        bool bulk_seen = true;
         for (int64_t j = 0; j + 1 < nt; ++j) {
             const int64_t g = j / AG, E = std::min<int64_t>((g + 1) * AG, nt);
             const int64_t bs1 = std::min<int64_t>(nb, n - (j + 1) * nb);
             const int64_t split = (bs1 + MPC_SYRK_TILE - 1) / MPC_SYRK_TILE;
-            if (j > 0) MPC_HIP(hipStreamWaitEvent(ps, h->ev_bulk, 0));    // st's last update
+            if (!bulk_seen) {                                              // st's last update
+                MPC_HIP(hipStreamWaitEvent(ps, h->ev_bulk, 0));
+                bulk_seen = true;
+            }
             if (j + 1 < E) {
                 // inside the group: block column j+1, then the group's other columns
                 if ((rc = update(j, 0, 2 * split, ps, 1))) return rc;
-                MPC_HIP(hipEventRecord(h->ev_panel, ps));
-                MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
                 const int64_t wend = (E - (j + 1)) * nb / MPC_SYRK_TILE;
-                if ((rc = update(j, split, wend, st, 0))) return rc;
-                MPC_HIP(hipEventRecord(h->ev_bulk, st));
+                if (wend > split) {
+                    MPC_HIP(hipEventRecord(h->ev_panel, ps));
+                    MPC_HIP(hipStreamWaitEvent(st, h->ev_panel, 0));
+                    if ((rc = update(j, split, wend, st, 0))) return rc;
+                    MPC_HIP(hipEventRecord(h->ev_bulk, st));
+                    bulk_seen = false;
+                }
             } else {
                 // the group's last panel: aggregated update beyond the group
                 const int64_t p0 = g * AG;
@@ -696,6 +706,7 @@ static int factor_once(mpc_handle* h, float* dA, int64_t n, int64_t lda, const m
                 if ((rc = agg(p0, np, E, nb / MPC_SYRK_TILE, (int64_t)AG * nb / MPC_SYRK_TILE, st, 2)))
                     return rc;
                 MPC_HIP(hipEventRecord(h->ev_bulk, st));
+                bulk_seen = false;
                 if ((rc = agg(p0, np, E, (int64_t)AG * nb / MPC_SYRK_TILE, -1, fs, 2))) return rc;
                 MPC_HIP(hipEventRecord(h->ev_far, fs));
                 far_pending = true;
