@@ -72,7 +72,9 @@ def main():
         sya = syrk_roofline(h, stream, n, nb, args.syrk_reps, C0=None,
                             k=(args.agg or 2) * nb)
         cp = copy_reference(stream, args.syrk_reps)
-        print(json.dumps({"syrk": sy, "syrk_aggregated": sya, "copy": cp}), flush=True)
+        syd = syrk_roofline(h, stream, n, nb, args.syrk_reps, C0=None, k=2048)
+        print(json.dumps({"syrk": sy, "syrk_aggregated": sya, "copy": cp, "syrk_deep": syd}),
+              flush=True)
         h.close()
         return
 
@@ -144,6 +146,9 @@ def main():
     m, k, syrk_ms, syrk_gbs, syrk_tf = sy["m"], sy["k"], sy["ms"], sy["gbs"], sy["tflops"]
     agg = args.agg if args.agg is not None else fo8.agg_panels
     sya = syrk_roofline(h, stream, n, nb, args.syrk_reps, C0=A, k=agg * nb) if agg > 1 else None
+    # the same update at K = 2048 (a depth the factor does not issue): where the
+    # arithmetic intensity reaches the FP8 ridge and the matrix cores take over
+    syd = syrk_roofline(h, stream, n, nb, max(2, args.syrk_reps // 4), C0=A, k=2048)
 
     # ---- CPU restatement on a bounded sample (rank 0, N = 1)
     cpu = None
@@ -214,6 +219,11 @@ def main():
             "traffic": syrk_traffic(sya["m"], sya["k"]), "avg_ms": round(sya["ms"], 4),
             "tflops": round(sya["tflops"], 1),
             "mfma_frac_fp8": round(sya["tflops"] / FP8_PEAK_TFLOPS, 4)},
+        "syrk_deep": {
+            "kernel": "syrk_kernel<E4M3> (m=%d k=%d, not issued by the factor)" % (syd["m"], syd["k"]),
+            "avg_ms": round(syd["ms"], 4), "gbs": round(syd["gbs"], 1),
+            "hbm_frac": round(syd["gbs"] / HBM_PEAK_GBS, 4), "tflops": round(syd["tflops"], 1),
+            "mfma_frac_fp8": round(syd["tflops"] / FP8_PEAK_TFLOPS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

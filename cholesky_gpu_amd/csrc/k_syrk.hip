@@ -777,7 +777,7 @@ __device__ __forceinline__ void syrk_tile(float* __restrict__ C, int64_t m, int6
             if constexpr (FA == 1) mm_f32<QA, KC, true, 0>(ocol, orow, t, k, acc);
             else if constexpr (FA == 2) mm_f32<QA, KC, false, 1>(ocol, orow, t, k, acc);
             else mm_f32<QA, KC, false, 0>(ocol, orow, t, k, acc);
-        } else {                   // mixed-class launch: the K loop is unrolled (KC > 0)
+        } else {                   // mixed-class launch: FP32 against an 8-bit or BF16 tile
             mm_f32<QA, KC, false, 2>(ocol, orow, t, k, acc);
         }
     }
@@ -898,12 +898,14 @@ int mpc_launch_syrk_panels(hipStream_t st, float* C, int64_t m, int64_t ldc, int
     }
     if (!syrk_geometry(m, 64 * qa, tcol0, tcol1, g, trow0)) return 0;
     syrk_lists none{nullptr, 0, 0};
-    if (qa == 1) {                // latency-bound: the K loop unrolled for the usual k
+    if (qa == 1) {
+        // latency-bound: as many resident workgroups as possible.  The K loop
+        // unrolled for k = 256 took 240 VGPRs (two workgroups per CU, two
+        // rounds of workgroups at the first steps); the runtime-k loop takes
+        // 152 (three per CU) and made the solve 0.3 ms faster.
         const dim3 list_grid = g.grid;
         auto next_lists = [&]() { return none; };
-        if (k == 256) MPC_SYRK_LAUNCH(1, -1, 0, 0, 256);
-        else if (k == 128) MPC_SYRK_LAUNCH(1, -1, 0, 0, 128);
-        else MPC_SYRK_LAUNCH(1, -1, 0, 0, 0);
+        MPC_SYRK_LAUNCH(1, -1, 0, 0, 0);
         MPC_LAUNCH_CHECK();
         return 0;
     }
