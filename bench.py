@@ -149,6 +149,8 @@ def main():
     # the same update at K = 2048 (a depth the factor does not issue): where the
     # arithmetic intensity reaches the FP8 ridge and the matrix cores take over
     syd = syrk_roofline(h, stream, n, nb, max(2, args.syrk_reps // 4), C0=A, k=2048)
+    # the streaming rate this device attains: a 1 GiB elementwise copy on the same stream
+    cpy = copy_reference(stream, max(2, args.syrk_reps // 2))
 
     # ---- CPU restatement on a bounded sample (rank 0, N = 1)
     cpu = None
@@ -210,7 +212,9 @@ def main():
                      "unit": "GB/s", "frac": round(syrk_gbs / HBM_PEAK_GBS, 4),
                      "traffic": syrk_traffic(m, k),
                      "avg_ms": round(syrk_ms, 4), "tflops": round(syrk_tf, 1),
-                     "mfma_frac_fp8": round(syrk_tf / FP8_PEAK_TFLOPS, 4)},
+                     "mfma_frac_fp8": round(syrk_tf / FP8_PEAK_TFLOPS, 4),
+                     "copy_gbs": round(cpy["gbs"], 1),
+                     "frac_of_copy": round(syrk_gbs / cpy["gbs"], 4)},
         "roofline_aggregated": None if sya is None else {
             "kernel": "syrk_kernel<E4M3> (the factor's aggregated update, m=%d k=%d)"
                       % (sya["m"], sya["k"]),
