@@ -177,13 +177,6 @@ constexpr int BW_CPW = 8;   // backward: columns per wave (32 per workgroup)
 constexpr int BW_RW = 512;  // backward: rows per split (BW_RPL per lane)
 constexpr int BW_RPL = BW_RW / 64;
 
-__device__ __forceinline__ double rdlane_d(double v, int l) {
-    const uint64_t b = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, l);
-    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), l);
-    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
-}
-
 // Forward, big block I = rows [p0, p0 + bI):
 //   part[(s nrhs + j) bI + i] = sum_{c in split s} L[p0 + i][c] Y[c][j],  c < p0
 // 64 rows per workgroup (lane = row: each column load is a 256-byte
@@ -312,17 +305,28 @@ __global__ __launch_bounds__(256) void bwd_gemv_partial_kernel(
 // Triangular GEMV on a diagonal big block, out[i] = sum_j M[i + j*ld] v[j]
 // over j <= i (lower: M = Winv, forward) or j >= i (upper: M = Winv^T,
 // backward), i, j < bI, where v[j] = V[j] - sum_{s < S} pin[(s nrhs + r) bI + j]
-// (the off-diagonal GEMV's partial sums, added in split order).  One wave per
-// 64 x 64 tile (row block ib, column chunk jb) of the triangle; lane = row;
-// the tile's 64 column loads are all in flight at once.  Partial sums go to
+// (the off-diagonal GEMV's partial sums).  One workgroup of TT_NW waves per
+// 64 x 64 tile (row block ib, column chunk jb) of the triangle; lane = row.
+// Wave w adds the partial sums s = w, w + TT_NW, .. of the tile's 64 v
+// entries (batches of 8 independent loads); the waves' sums are combined in
+// LDS in wave order.  Wave w then multiplies columns [16 w, 16 w + 16) of the
+// tile (its 16 loads in flight at once, v read back from LDS) and the four
+// products are added in wave order.  Partial sums go to
 // part[(jb * nrhs + j) * bI + i]; tri_gemv_reduce adds the chunks of a row
 // in order (deterministic).
+constexpr int TT_NW = 4;
+constexpr int TT_CW = TG / TT_NW;          // tile columns per wave
 template <bool UPPER>
-__global__ __launch_bounds__(64) void tri_gemv_tile_kernel(const float* __restrict__ M, int ld,
-                                                          int bI, const double* __restrict__ V,
-                                                          int64_t ldv, int nrhs,
-                                                          const double* __restrict__ pin, int S,
-                                                          double* __restrict__ part) {
+__global__ __launch_bounds__(64 * TT_NW) void tri_gemv_tile_kernel(const float* __restrict__ M,
+                                                                  int ld, int bI,
+                                                                  const double* __restrict__ V,
+                                                                  int64_t ldv, int nrhs,
+                                                                  const double* __restrict__ pin,
+                                                                  int S,
+                                                                  double* __restrict__ part) {
+    __shared__ double vsum[TT_NW][64];
+    __shared__ double vs[64];
+    __shared__ double acc_w[TT_NW][64];
     // tile t -> (ib, jb): lower ib >= jb, t = ib (ib + 1) / 2 + jb; upper mirrors it
     const int t = blockIdx.x;
     int a = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
@@ -330,66 +334,78 @@ __global__ __launch_bounds__(64) void tri_gemv_tile_kernel(const float* __restri
     while ((a + 1) * (a + 2) / 2 <= t) ++a;
     const int b = t - a * (a + 1) / 2;
     const int ib = UPPER ? b : a, jb = UPPER ? a : b;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int i = ib * TG + lane;
     const int j0 = jb * TG;
+    const int c0 = j0 + TT_CW * wave;      // this wave's first column
     const int r0 = blockIdx.y * NR;
     const int nj = nrhs - r0 < NR ? nrhs - r0 : NR;
-    // All 64 loads of the tile issued before any use.  A load guarded by a
-    // condition (or whose use is) would be sunk into a branch with its own
-    // wait, serialising the loads; hence three shapes of tile:
-    float m[TG];
+    // This wave's 16 columns of the tile, all loads issued before any use.  A
+    // load guarded by a condition (or whose use is) would be sunk into a
+    // branch with its own wait, serialising the loads; hence three shapes:
+    float m[TT_CW];
     const bool full = j0 + TG <= bI && ib * TG + TG <= bI;
     if (full && ib != jb) {                  // interior: no mask
-        const float* Mi = M + i + (int64_t)j0 * ld;
+        const float* Mi = M + i + (int64_t)c0 * ld;
 #pragma unroll
-        for (int u = 0; u < TG; ++u) m[u] = Mi[(int64_t)u * ld];
+        for (int u = 0; u < TT_CW; ++u) m[u] = Mi[(int64_t)u * ld];
     } else if (full) {                       // diagonal tile: per-lane triangle select
-        const float* Mi = M + i + (int64_t)j0 * ld;
+        const float* Mi = M + i + (int64_t)c0 * ld;
 #pragma unroll
-        for (int u = 0; u < TG; ++u) m[u] = Mi[(int64_t)u * ld];
+        for (int u = 0; u < TT_CW; ++u) m[u] = Mi[(int64_t)u * ld];
 #pragma unroll
-        for (int u = 0; u < TG; ++u) {
-            const bool keep = UPPER ? u >= lane : u <= lane;
+        for (int u = 0; u < TT_CW; ++u) {
+            const int cl = TT_CW * wave + u;              // column within the tile
+            const bool keep = UPPER ? cl >= lane : cl <= lane;
             m[u] = keep ? m[u] : 0.f;
         }
     } else {                                 // ragged last block: clamped addresses
         const int ic = i < bI ? i : bI - 1;
 #pragma unroll
-        for (int u = 0; u < TG; ++u) {
-            const int j = j0 + u;
+        for (int u = 0; u < TT_CW; ++u) {
+            const int j = c0 + u;
             m[u] = M[ic + (int64_t)(j < bI ? j : bI - 1) * ld];
         }
 #pragma unroll
-        for (int u = 0; u < TG; ++u) {
-            const int j = j0 + u;
+        for (int u = 0; u < TT_CW; ++u) {
+            const int j = c0 + u;
             const bool keep = j < bI && (UPPER ? j >= i : j <= i);
             m[u] = keep ? m[u] : 0.f;
         }
     }
-    // lane u builds v[j0 + u]; the products read it back by lane index
+    // lane u of every wave builds a part of v[j0 + u]
     const int jl = j0 + lane < bI ? j0 + lane : bI - 1;
+    const int64_t sstride = (int64_t)nrhs * bI;
     for (int r = 0; r < nj; ++r) {
-        double sum = 0.0;
         const double* pr = pin + (int64_t)(r0 + r) * bI + jl;
-        const int64_t sstride = (int64_t)nrhs * bI;
-        int sp = 0;
-        for (; sp + 8 <= S; sp += 8) {       // batches of 8 independent loads
+        double sum = 0.0;
+        int sp = wave;
+        for (; sp + 7 * TT_NW < S; sp += 8 * TT_NW) {      // batches of 8 independent loads
             double pv[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) pv[q] = pr[(sp + q) * sstride];
+            for (int q = 0; q < 8; ++q) pv[q] = pr[(int64_t)(sp + q * TT_NW) * sstride];
 #pragma unroll
             for (int q = 0; q < 8; ++q) sum += pv[q];
         }
-        for (; sp < S; ++sp) sum += pr[sp * sstride];
-        const double vv = V[(int64_t)(r0 + r) * ldv + jl] - sum;
-        double acc0 = 0.0, acc1 = 0.0;
+        for (; sp < S; sp += TT_NW) sum += pr[(int64_t)sp * sstride];
+        vsum[wave][lane] = sum;
+        __syncthreads();
+        if (wave == 0)
+            vs[lane] = V[(int64_t)(r0 + r) * ldv + jl] -
+                       (((vsum[0][lane] + vsum[1][lane]) + vsum[2][lane]) + vsum[3][lane]);
+        __syncthreads();
+        double a0 = 0.0, a1 = 0.0;
 #pragma unroll
-        for (int u = 0; u < TG; u += 2) {
-            acc0 = fma((double)m[u], rdlane_d(vv, u), acc0);
-            acc1 = fma((double)m[u + 1], rdlane_d(vv, u + 1), acc1);
+        for (int u = 0; u < TT_CW; u += 2) {
+            a0 = fma((double)m[u], vs[TT_CW * wave + u], a0);
+            a1 = fma((double)m[u + 1], vs[TT_CW * wave + u + 1], a1);
         }
-        if (i < bI) part[((int64_t)jb * nrhs + r0 + r) * bI + i] = acc0 + acc1;
+        acc_w[wave][lane] = a0 + a1;
+        __syncthreads();
+        if (wave == 0 && i < bI)
+            part[((int64_t)jb * nrhs + r0 + r) * bI + i] =
+                ((acc_w[0][lane] + acc_w[1][lane]) + acc_w[2][lane]) + acc_w[3][lane];
+        __syncthreads();                    // vsum, vs and acc_w are reused
     }
 }
 
@@ -535,11 +551,11 @@ static int tri_gemv(hipStream_t st, const float* M, int Bo, int bI, int upper, c
     const int nb = (bI + TG - 1) / TG;
     const dim3 grid((unsigned)(nb * (nb + 1) / 2), cdiv(nrhs, NR));
     if (upper)
-        hipLaunchKernelGGL(tri_gemv_tile_kernel<true>, grid, dim3(64), 0, st, M, Bo, bI, V, ldv,
-                           nrhs, pin, S, part);
+        hipLaunchKernelGGL(tri_gemv_tile_kernel<true>, grid, dim3(64 * TT_NW), 0, st, M, Bo, bI, V,
+                           ldv, nrhs, pin, S, part);
     else
-        hipLaunchKernelGGL(tri_gemv_tile_kernel<false>, grid, dim3(64), 0, st, M, Bo, bI, V, ldv,
-                           nrhs, pin, S, part);
+        hipLaunchKernelGGL(tri_gemv_tile_kernel<false>, grid, dim3(64 * TT_NW), 0, st, M, Bo, bI, V,
+                           ldv, nrhs, pin, S, part);
     hipLaunchKernelGGL(tri_gemv_reduce_kernel, dim3(cdiv(bI, 64), (unsigned)nrhs), dim3(256), 0, st,
                        part, bI, nrhs, upper, out, ldo);
     MPC_LAUNCH_CHECK();
