@@ -309,12 +309,15 @@ __global__ __launch_bounds__(256) void bwd_gemv_partial_kernel(
 // 64 x 64 tile (row block ib, column chunk jb) of the triangle; lane = row.
 // Wave w adds the partial sums s = w, w + TT_NW, .. of the tile's 64 v
 // entries (batches of 8 independent loads); the waves' sums are combined in
-// LDS in wave order.  Wave w then multiplies columns [16 w, 16 w + 16) of the
-// tile (its 16 loads in flight at once, v read back from LDS) and the four
+// LDS in wave order.  Wave w then multiplies its TT_CW columns of the tile
+// (their loads in flight at once, v read back from LDS) and the waves'
 // products are added in wave order.  Partial sums go to
 // part[(jb * nrhs + j) * bI + i]; tri_gemv_reduce adds the chunks of a row
 // in order (deterministic).
-constexpr int TT_NW = 4;
+#ifndef MPC_TT_NW
+#define MPC_TT_NW 4
+#endif
+constexpr int TT_NW = MPC_TT_NW;          // waves per tile
 constexpr int TT_CW = TG / TT_NW;          // tile columns per wave
 template <bool UPPER>
 __global__ __launch_bounds__(64 * TT_NW) void tri_gemv_tile_kernel(const float* __restrict__ M,
@@ -390,9 +393,12 @@ __global__ __launch_bounds__(64 * TT_NW) void tri_gemv_tile_kernel(const float* 
         for (; sp < S; sp += TT_NW) sum += pr[(int64_t)sp * sstride];
         vsum[wave][lane] = sum;
         __syncthreads();
-        if (wave == 0)
-            vs[lane] = V[(int64_t)(r0 + r) * ldv + jl] -
-                       (((vsum[0][lane] + vsum[1][lane]) + vsum[2][lane]) + vsum[3][lane]);
+        if (wave == 0) {
+            double tot = vsum[0][lane];
+#pragma unroll
+            for (int w = 1; w < TT_NW; ++w) tot += vsum[w][lane];
+            vs[lane] = V[(int64_t)(r0 + r) * ldv + jl] - tot;
+        }
         __syncthreads();
         double a0 = 0.0, a1 = 0.0;
 #pragma unroll
@@ -402,9 +408,12 @@ __global__ __launch_bounds__(64 * TT_NW) void tri_gemv_tile_kernel(const float* 
         }
         acc_w[wave][lane] = a0 + a1;
         __syncthreads();
-        if (wave == 0 && i < bI)
-            part[((int64_t)jb * nrhs + r0 + r) * bI + i] =
-                ((acc_w[0][lane] + acc_w[1][lane]) + acc_w[2][lane]) + acc_w[3][lane];
+        if (wave == 0 && i < bI) {
+            double tot = acc_w[0][lane];
+#pragma unroll
+            for (int w = 1; w < TT_NW; ++w) tot += acc_w[w][lane];
+            part[((int64_t)jb * nrhs + r0 + r) * bI + i] = tot;
+        }
         __syncthreads();                    // vsum, vs and acc_w are reused
     }
 }
